@@ -1,0 +1,153 @@
+"""Benchmark of the bit-exact bisulfite ksw_extend2 path on one MI355X.
+
+Workload (default cfg1): a 50 Mb synthetic bisulfite genome, 1M paired-end
+2x150 read pairs simulated on it, and every ksw_extend2 call mem_chain2aln
+(bsbolt/External/BWA/bwamem.c:636-790) issues for them -- left/right flanks of
+the seeds, right-flank h0 = left score, w<<1 retries -- built in rounds on the
+GPU.  A step is one pass of the extension kernels over that whole batch, with
+inputs and outputs resident in device memory.
+
+Prints ONE JSON line: extensions/s (value), GCUPS, the integer-VALU roofline
+of the extension kernels, their HBM figure, and the CPU baseline: the
+reference's own ksw_extend2 (compiled from bsbolt/External/BWA/ksw.c) run over
+the same batch on the host cores, with its outputs compared to the GPU's.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X (gfx950): 256 CUs, 2.4 GHz peak engine clock, 64 int32 lanes/CU/clock.
+# One extension cell is 12 integer VALU operations in the reference inner loop
+# (ksw.c:433-447: 1 add + 1 select for M, 2 max for H, 2 sub + 2 max each for E and F).
+OPS_PER_CELL = 12
+PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12          # 39.3 T int32 ops/s
+PEAK_PK16_TOPS = 2 * PEAK_INT32_TOPS               # 78.6 T ops/s with packed 16-bit VALU
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cfg1")
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload's reads")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the extension batch shards "
+                                   "by task index with no exchange (see DESIGN.md), multi-GPU runs are "
+                                   "not part of this build", "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    from bsbolt_amd import bsx
+    from bsbolt_amd.workload import CONFIGS, Workload
+    from bsbolt_amd.tasks import RESULT_FIELDS
+
+    cfg = CONFIGS[args.workload]
+    if args.scale != 1.0:
+        cfg = cfg.scaled(args.scale)
+    eng = bsx.Engine(0)
+    t0 = time.time()
+    wl = Workload(cfg)
+    log(f"simulated {wl.n_reads} reads on a {cfg.genome_len / 1e6:.0f} Mb genome in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    batch, round_res, side, rnd = wl.build_ext_batch(lambda t: eng.extend(t, cells=False),
+                                                     on_round=lambda r, n: log(f"round {r}: {n} tasks"))
+    wl.close()
+    n = len(batch)
+    log(f"built {n} extension tasks in {time.time() - t0:.1f}s")
+
+    # resident batch
+    keep, ptrs = eng.tasks_to_device(batch)
+    outs = {k: eng.empty((n,), np.int32) for k in RESULT_FIELDS}
+    cells_d = eng.empty((n,), np.int64)
+    ptrs.update({k: v.ptr for k, v in outs.items()})
+    ptrs["cells"] = cells_d.ptr
+    ptrs["n"] = n
+    for _ in range(args.warmup):
+        eng.extend_device(ptrs)
+    eng.sync()
+    got = {k: v.numpy() for k, v in outs.items()}
+    cells = cells_d.numpy()
+    for k in RESULT_FIELDS:  # the resident run must agree with the rounds that built the batch
+        if not np.array_equal(got[k], round_res[k]):
+            raise SystemExit(f"resident run disagrees with the build rounds on {k}")
+    total_cells = int(cells.sum())
+    nominal = int((batch.qlen.astype(np.int64) * batch.tlen).sum())
+
+    # timed region: K steps bracketed by stream synchronisation; HIP events on
+    # the stream the kernels run on
+    eng.sync()
+    w0 = time.perf_counter()
+    eng.timer_start()
+    for _ in range(args.steps):
+        eng.extend_device(ptrs)
+    ms = eng.timer_stop()
+    eng.sync()
+    wall = time.perf_counter() - w0
+    ms_step = ms / args.steps
+    ext_s = n / (ms_step / 1e3)
+    gcups = total_cells / (ms_step / 1e3) / 1e9
+    tops = total_cells * OPS_PER_CELL / (ms_step / 1e3) / 1e12
+    alg_bytes = int(batch.qlen.astype(np.int64).sum() + batch.tlen.astype(np.int64).sum()) + n * (36 + 24 + 8)
+    gbs = alg_bytes / (ms_step / 1e3) / 1e9
+    log(f"{args.steps} steps: {ms_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} ms/step")
+
+    line = {
+        "metric": "extensions/s + GCUPS for bit-exact bisulfite ksw_extend2; learned-band vs plain",
+        "value": ext_s,
+        "unit": "extensions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded genome + bisulfite read simulation; no external data)",
+        "config": {"workload": f"{args.workload}: {cfg.genome_len // 1_000_000} Mb genome, {wl.n_reads} reads "
+                               f"x {cfg.read_len} bp{' PE' if cfg.paired else ' SE'}, all mem_chain2aln "
+                               f"ksw_extend2 calls", "tasks": n, "form": "plain",
+                   "cells_ref": total_cells, "cells_nominal": nominal,
+                   "mean_qlen": float(batch.qlen.mean()), "mean_tlen": float(batch.tlen.mean())},
+        "gcups": gcups,
+        "gcups_nominal": nominal / (ms_step / 1e3) / 1e9,
+        "roofline": {"bound": "valu", "achieved": tops, "peak": PEAK_PK16_TOPS, "unit": "TOP/s",
+                     "frac": tops / PEAK_PK16_TOPS, "frac_int32": tops / PEAK_INT32_TOPS, "traffic": None,
+                     "ops_per_cell": OPS_PER_CELL},
+        "roofline_hbm": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes": alg_bytes},
+        "learned_band": None,
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        kind = "reference" if O.ref_available() else "port"
+        log(f"CPU baseline: {kind} ksw_extend2 over all {n} tasks on {args.cpu_threads} threads")
+        cpu_out, dt = O.extend2_batch(batch, nthreads=args.cpu_threads, use_ref=(kind == "reference"),
+                                      want_cells=False)
+        mism = int(sum(int((cpu_out[k] != got[k]).sum()) for k in RESULT_FIELDS))
+        line["cpu_baseline"] = {"value": n / dt, "unit": "extensions/s", "cores": args.cpu_threads,
+                                "kind": kind, "gcups": total_cells / dt / 1e9, "seconds": dt,
+                                "sample": f"the full batch ({n} tasks), dynamic 4096-task chunks per thread",
+                                "mismatches_vs_gpu": mism}
+        line["speedup_vs_cpu"] = ext_s / (n / dt)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

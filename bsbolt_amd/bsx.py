@@ -1,0 +1,314 @@
+"""Python host of libbsx.so (ctypes), the role bsbolt.Align plays for the
+reference aligner, with the operator interface of the reference kernels:
+
+  ksw_extend2(qlen, query, tlen, target, m, mat, o_del, e_del, o_ins, e_ins,
+              w, end_bonus, zdrop, h0) -> (score, qle, tle, gtle, gscore, max_off)
+      bsbolt/External/BWA/ksw.h:108, ksw.c:380-479
+  ksw_global2(qlen, query, tlen, target, m, mat, o_del, e_del, o_ins, e_ins, w)
+      -> (score, n_cigar, cigar)           ksw.h:84, ksw.c:504-606
+
+plus the batched forms the GPU is built for (Engine.extend / Engine.global_).
+Every call runs on the GPU through the C ABI; there is no CPU fallback: if the
+library or a GPU is missing, loading fails with an error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .tasks import ExtTasks, GlbTasks, RESULT_FIELDS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbsx.so")
+
+BSX_OK, BSX_EINVAL, BSX_EDEVICE, BSX_ENOMEM, BSX_ERANGE = 0, -1, -2, -3, -4
+BSX_MAX_QLEN = 2048
+_ERRNAME = {BSX_EINVAL: "BSX_EINVAL", BSX_EDEVICE: "BSX_EDEVICE", BSX_ENOMEM: "BSX_ENOMEM",
+            BSX_ERANGE: "BSX_ERANGE"}
+
+# Every symbol include/bsx.h declares.
+EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx_sync", "bsx_version",
+           "bsx_default_scoring", "bsx_extend_batch", "bsx_extend_batch_device", "bsx_global_batch",
+           "bsx_global_batch_device", "bsx_set_global_zslot", "bsx_ksw_extend2", "bsx_dev_alloc",
+           "bsx_dev_free", "bsx_memcpy", "bsx_timer_start", "bsx_timer_stop")
+
+
+class BsxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRNAME.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Scoring(C.Structure):
+    _fields_ = [("mat", C.c_int8 * 25), ("o_del", C.c_int32), ("e_del", C.c_int32),
+                ("o_ins", C.c_int32), ("e_ins", C.c_int32), ("zdrop", C.c_int32)]
+
+    @classmethod
+    def make(cls, mat=None, o_del=6, e_del=1, o_ins=6, e_ins=1, zdrop=100, a=1, b=4):
+        s = cls()
+        m = fill_scmat(a, b) if mat is None else np.asarray(mat, np.int8).reshape(25)
+        for k in range(25):
+            s.mat[k] = int(m[k])
+        s.o_del, s.e_del, s.o_ins, s.e_ins, s.zdrop = o_del, e_del, o_ins, e_ins, zdrop
+        return s
+
+    def as_dict(self):
+        return dict(mat=np.array(list(self.mat), np.int8), o_del=self.o_del, e_del=self.e_del,
+                    o_ins=self.o_ins, e_ins=self.e_ins, zdrop=self.zdrop)
+
+
+class _ExtTasks(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in
+                                     ("query", "q_off", "qlen", "target", "t_off", "tlen", "h0", "w",
+                                      "end_bonus")]
+
+
+class _ExtResults(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in RESULT_FIELDS + ("cells",)]
+
+
+class _GlbTasks(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in
+                                     ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")]
+
+
+class _GlbResults(C.Structure):
+    _fields_ = [("score", C.c_void_p), ("n_cigar", C.c_void_p), ("cigar", C.c_void_p),
+                ("cigar_slot", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libbsx.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libbsx.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.bsx_create.restype = C.c_void_p
+        L.bsx_create.argtypes = [C.c_int]
+        L.bsx_destroy.argtypes = [C.c_void_p]
+        L.bsx_last_error.restype = C.c_char_p
+        L.bsx_last_error.argtypes = [C.c_void_p]
+        L.bsx_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+        L.bsx_sync.argtypes = [C.c_void_p]
+        L.bsx_version.restype = C.c_char_p
+        L.bsx_default_scoring.argtypes = [C.POINTER(Scoring)]
+        for f, tk, rs in (("bsx_extend_batch", _ExtTasks, _ExtResults),
+                          ("bsx_extend_batch_device", _ExtTasks, _ExtResults),
+                          ("bsx_global_batch", _GlbTasks, _GlbResults),
+                          ("bsx_global_batch_device", _GlbTasks, _GlbResults)):
+            getattr(L, f).argtypes = [C.c_void_p, C.POINTER(Scoring), C.POINTER(tk), C.POINTER(rs)]
+            getattr(L, f).restype = C.c_int
+        L.bsx_set_global_zslot.argtypes = [C.c_void_p, C.c_int64]
+        L.bsx_dev_alloc.restype = C.c_void_p
+        L.bsx_dev_alloc.argtypes = [C.c_void_p, C.c_size_t]
+        L.bsx_dev_free.argtypes = [C.c_void_p, C.c_void_p]
+        L.bsx_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.bsx_timer_start.argtypes = [C.c_void_p]
+        L.bsx_timer_stop.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+        P = C.c_void_p
+        L.bsx_ksw_extend2.restype = C.c_int
+        L.bsx_ksw_extend2.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P] + [C.c_int] * 8 + [P] * 5
+        _lib = L
+    return _lib
+
+
+def fill_scmat(a=1, b=4):
+    """bwa_fill_scmat (bsbolt/External/BWA/bwa.c:169-178)."""
+    m = np.full((5, 5), -b, np.int8)
+    np.fill_diagonal(m, a)
+    m[4, :] = -1
+    m[:, 4] = -1
+    return m.reshape(25)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Engine:
+    """A libbsx context on one GPU (one HIP stream)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.ctx = self.L.bsx_create(device)
+        if not self.ctx:
+            raise RuntimeError(f"bsx_create({device}) failed: no usable GPU")
+
+    def close(self):
+        if self.ctx:
+            self.L.bsx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != BSX_OK:
+            raise BsxError(rc, self.L.bsx_last_error(self.ctx).decode())
+
+    def last_error(self):
+        return self.L.bsx_last_error(self.ctx).decode()
+
+    def set_stream(self, stream_handle):
+        self._check(self.L.bsx_set_stream(self.ctx, stream_handle))
+
+    def sync(self):
+        self._check(self.L.bsx_sync(self.ctx))
+
+    def set_global_zslot(self, nbytes):
+        self.L.bsx_set_global_zslot(self.ctx, int(nbytes))
+
+    # ---- device memory ----
+    def to_device(self, arr):
+        """Copies a numpy array into new device memory; returns a DevArray."""
+        arr = np.ascontiguousarray(arr)
+        d = DevArray(self, arr.nbytes, arr.dtype, arr.shape)
+        if arr.nbytes:
+            self._check(self.L.bsx_memcpy(self.ctx, d.ptr, arr.ctypes.data, arr.nbytes, 1))
+        return d
+
+    def empty(self, shape, dtype):
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        return DevArray(self, n, np.dtype(dtype), shape)
+
+    def timer_start(self):
+        self._check(self.L.bsx_timer_start(self.ctx))
+
+    def timer_stop(self):
+        ms = C.c_float(0)
+        self._check(self.L.bsx_timer_stop(self.ctx, C.byref(ms)))
+        return ms.value
+
+    def tasks_to_device(self, tasks):
+        """Uploads an ExtTasks batch; returns (dict of DevArrays, dict of addresses)."""
+        keep = {k: self.to_device(getattr(tasks, k)) for k in
+                ("query", "q_off", "qlen", "target", "t_off", "tlen", "h0", "w", "end_bonus")}
+        return keep, {k: v.ptr for k, v in keep.items()}
+
+    # ---- extension ----
+    def extend(self, tasks: ExtTasks, scoring: Scoring = None, cells=True, out=None, raise_on_error=True):
+        """Host-memory batch of ksw_extend2 calls -> dict of int32 arrays
+        (score, qle, tle, gtle, gscore, max_off) and int64 cells."""
+        sc = scoring or Scoring.make()
+        n = len(tasks)
+        res = out if out is not None else {k: np.zeros(n, np.int32) for k in RESULT_FIELDS}
+        if cells and "cells" not in res:
+            res["cells"] = np.zeros(n, np.int64)
+        tk = _ExtTasks(n, *[_p(getattr(tasks, k)) for k in
+                            ("query", "q_off", "qlen", "target", "t_off", "tlen", "h0", "w", "end_bonus")])
+        rs = _ExtResults(*[_p(res.get(k)) for k in RESULT_FIELDS + ("cells",)])
+        rc = self.L.bsx_extend_batch(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs))
+        if raise_on_error:
+            self._check(rc)
+        return res if raise_on_error else (res, rc)
+
+    def extend_device(self, dev, scoring: Scoring = None):
+        """Device-memory batch: `dev` maps the task/result field names to
+        device addresses (e.g. torch tensors' data_ptr()).  Asynchronous; call
+        sync() to wait and collect precondition errors."""
+        sc = scoring or Scoring.make()
+        tk = _ExtTasks(dev["n"], *[dev.get(k) for k in
+                                   ("query", "q_off", "qlen", "target", "t_off", "tlen", "h0", "w",
+                                    "end_bonus")])
+        rs = _ExtResults(*[dev.get(k) for k in RESULT_FIELDS + ("cells",)])
+        self._check(self.L.bsx_extend_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
+
+    # ---- global ----
+    def global_(self, tasks: GlbTasks, scoring: Scoring = None, cigar=True, slot=None,
+                raise_on_error=True):
+        """Host-memory batch of ksw_global2 calls -> (score, n_cigar, cigar[n, slot])."""
+        sc = scoring or Scoring.make()
+        n = len(tasks)
+        if slot is None:
+            slot = int((tasks.qlen + tasks.tlen).max()) if n else 1
+        score = np.zeros(n, np.int32)
+        ncig = np.zeros(n, np.int32)
+        cig = np.zeros(n * slot, np.uint32) if cigar else None
+        tk = _GlbTasks(n, *[_p(getattr(tasks, k)) for k in
+                            ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")])
+        rs = _GlbResults(_p(score), _p(ncig), _p(cig), slot if cigar else 0)
+        rc = self.L.bsx_global_batch(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs))
+        out = (score, ncig, cig.reshape(n, slot) if cigar else None)
+        if raise_on_error:
+            self._check(rc)
+            return out
+        return out, rc
+
+    def global_device(self, dev, scoring: Scoring = None):
+        sc = scoring or Scoring.make()
+        tk = _GlbTasks(dev["n"], *[dev.get(k) for k in
+                                   ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")])
+        rs = _GlbResults(dev.get("score"), dev.get("n_cigar"), dev.get("cigar"), dev.get("cigar_slot", 0))
+        self._check(self.L.bsx_global_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
+
+
+class DevArray:
+    """Device memory owned through a libbsx context (no other HIP runtime is
+    needed in the process)."""
+
+    def __init__(self, eng, nbytes, dtype, shape):
+        self.eng, self.nbytes, self.dtype, self.shape = eng, nbytes, np.dtype(dtype), shape
+        self.ptr = eng.L.bsx_dev_alloc(eng.ctx, max(1, nbytes))
+        if not self.ptr:
+            raise BsxError(BSX_ENOMEM, eng.last_error())
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            self.eng._check(self.eng.L.bsx_memcpy(self.eng.ctx, out.ctypes.data, self.ptr, self.nbytes, 2))
+        return out
+
+    def free(self):
+        if self.ptr and self.eng.ctx:
+            self.eng.L.bsx_dev_free(self.eng.ctx, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def _engine():
+    global _default
+    if _default is None:
+        _default = Engine(0)
+    return _default
+
+
+def ksw_extend2(qlen, query, tlen, target, m, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, h0):
+    """One ksw_extend2 call on the GPU (argument meaning as ksw.h:84-108).
+    Returns (score, qle, tle, gtle, gscore, max_off).  Where the reference
+    asserts (h0 <= 0, ksw.c:385) this raises BsxError(BSX_EINVAL)."""
+    if m != 5:
+        raise BsxError(BSX_EINVAL, "m must be 5")
+    q = np.ascontiguousarray(np.asarray(query, np.uint8)[:qlen])
+    t = np.ascontiguousarray(np.asarray(target, np.uint8)[:tlen])
+    tk = ExtTasks.from_pairs([(q, t)], h0, w, end_bonus)
+    sc = Scoring.make(mat=mat, o_del=o_del, e_del=e_del, o_ins=o_ins, e_ins=e_ins, zdrop=zdrop)
+    r = _engine().extend(tk, sc, cells=False)
+    return tuple(int(r[k][0]) for k in RESULT_FIELDS)
+
+
+def ksw_global2(qlen, query, tlen, target, m, mat, o_del, e_del, o_ins, e_ins, w):
+    """One ksw_global2 call on the GPU.  Returns (score, n_cigar, [cigar uint32])."""
+    if m != 5:
+        raise BsxError(BSX_EINVAL, "m must be 5")
+    q = np.asarray(query, np.uint8)[:qlen]
+    t = np.asarray(target, np.uint8)[:tlen]
+    tk = GlbTasks.from_pairs([(q, t)], w)
+    sc = Scoring.make(mat=mat, o_del=o_del, e_del=e_del, o_ins=o_ins, e_ins=e_ins)
+    s, nc, cg = _engine().global_(tk, sc, cigar=True)
+    n = int(nc[0])
+    return int(s[0]), n, [int(x) for x in cg[0, :n]]

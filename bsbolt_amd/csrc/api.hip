@@ -1,0 +1,487 @@
+// C ABI of libbsx.so (declared in include/bsx.h): contexts, argument checks,
+// host<->device staging, and dispatch to the extension / global kernels.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bsx.h"
+#include "bsx_internal.h"
+
+using namespace bsx;
+
+namespace {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace
+
+struct bsx_ctx {
+  int device = 0;
+  int num_cu = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // extension workspace
+  DevBuf bin_of, list, small;  // small: counts/cursor/fetch/err
+  DevBuf zbuf;                 // direction-byte slots of the global kernel
+  int64_t zslot = 1 << 16;     // bytes per slot: largest min(qlen,2w+1)*tlen accepted
+  // staging for the host-memory entry points
+  DevBuf h_query, h_target, h_qoff, h_toff, h_i32, h_res;
+  int32_t* err_host = nullptr;  // pinned: [0]=rejected count, [1]=first rejected index
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool pending_check = false;
+  int pending_kind = 0;  // 1 = extension, 2 = global
+};
+
+static int fail(bsx_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+static int hip_fail(bsx_ctx* c, hipError_t e, const char* what) {
+  return fail(c, BSX_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr, what)                      \
+  do {                                             \
+    hipError_t _e = (expr);                        \
+    if (_e != hipSuccess) return hip_fail(c, _e, what); \
+  } while (0)
+
+static int check_scoring(bsx_ctx* c, const bsx_scoring* sc) {
+  if (!sc) return fail(c, BSX_EINVAL, "scoring is NULL");
+  const int lim = 1 << 16;
+  if (sc->e_del < 1 || sc->e_ins < 1 || sc->e_del > lim || sc->e_ins > lim)
+    return fail(c, BSX_EINVAL, "gap extension penalties must be in [1, 65536] (got %d, %d)", sc->e_del,
+                sc->e_ins);
+  if (sc->o_del < 0 || sc->o_ins < 0 || sc->o_del > lim || sc->o_ins > lim)
+    return fail(c, BSX_EINVAL, "gap open penalties must be in [0, 65536] (got %d, %d)", sc->o_del,
+                sc->o_ins);
+  return BSX_OK;
+}
+
+template <class Args>
+static void fill_profile(Args& a, const bsx_scoring* sc) {
+  for (int qc = 0; qc < 5; ++qc) {
+    uint32_t lo = 0;
+    for (int t = 0; t < 4; ++t) lo |= (uint32_t)(uint8_t)sc->mat[t * 5 + qc] << (8 * t);
+    a.prof_lo[qc] = lo;
+    a.prof_hi[qc] = sc->mat[20 + qc];
+  }
+}
+
+static int mat_max(const bsx_scoring* sc) {
+  int m = 0;  // ksw.c:400-401 starts the maximum at 0
+  for (int k = 0; k < 25; ++k) m = sc->mat[k] > m ? sc->mat[k] : m;
+  return m;
+}
+
+extern "C" {
+
+const char* bsx_version(void) { return "bsx 0.1 (gfx950)"; }
+
+void bsx_default_scoring(bsx_scoring* sc) {
+  // bwa_fill_scmat(a=1, b=4) (bwa.c:169-178); BSBolt defaults -O 6,6 -E 1,1 -d 100
+  for (int r = 0; r < 5; ++r)
+    for (int q = 0; q < 5; ++q) sc->mat[r * 5 + q] = (r == 4 || q == 4) ? -1 : (r == q ? 1 : -4);
+  sc->o_del = sc->o_ins = 6;
+  sc->e_del = sc->e_ins = 1;
+  sc->zdrop = 100;
+}
+
+bsx_ctx* bsx_create(int device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  bsx_ctx* c = new bsx_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return nullptr; }
+  c->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+  c->stream = c->own;
+  if (hipHostMalloc((void**)&c->err_host, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return nullptr;
+  }
+  c->err_host[0] = 0;
+  c->err_host[1] = INT_MAX;
+  c->err_host[2] = 0;
+  return c;
+}
+
+void bsx_destroy(bsx_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->bin_of, &c->list, &c->small, &c->zbuf, &c->h_query, &c->h_target, &c->h_qoff, &c->h_toff,
+                    &c->h_i32, &c->h_res})
+    b->release();
+  if (c->err_host) (void)hipHostFree(c->err_host);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
+const char* bsx_last_error(const bsx_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int bsx_set_stream(bsx_ctx* c, void* s) {
+  if (!c) return BSX_EINVAL;
+  c->stream = s ? (hipStream_t)s : c->own;
+  return BSX_OK;
+}
+
+int bsx_sync(bsx_ctx* c) {
+  if (!c) return BSX_EINVAL;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (c->pending_check) {
+    c->pending_check = false;
+    if (c->err_host[0] > 0)
+      return fail(c, BSX_EINVAL, "%d task(s) rejected (precondition violated); first is task %d",
+                  c->err_host[0], c->err_host[1]);
+    if (c->pending_kind == 2 && c->err_host[2] > 0)
+      return fail(c, BSX_ERANGE, "%d CIGAR(s) did not fit their slot", c->err_host[2]);
+  }
+  return BSX_OK;
+}
+
+int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t,
+                            bsx_ext_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  int rc = check_scoring(c, sc);
+  if (rc) return rc;
+  if (t->n < 0 || t->n > INT_MAX) return fail(c, BSX_EINVAL, "n must be in [0, 2^31-1] (got %lld)", (long long)t->n);
+  if (t->n > 0 && (!t->query || !t->q_off || !t->qlen || !t->target || !t->t_off || !t->tlen || !t->h0 ||
+                   !t->w || !t->end_bonus))
+    return fail(c, BSX_EINVAL, "task arrays must not be NULL");
+  (void)hipSetDevice(c->device);
+  ExtArgs a{};
+  a.o_del = sc->o_del; a.e_del = sc->e_del; a.o_ins = sc->o_ins; a.e_ins = sc->e_ins;
+  a.zdrop = sc->zdrop;
+  a.mat_max = mat_max(sc);
+  fill_profile(a, sc);
+  a.n = t->n;
+  a.query = t->query; a.q_off = t->q_off; a.qlen = t->qlen;
+  a.target = t->target; a.t_off = t->t_off; a.tlen = t->tlen;
+  a.h0 = t->h0; a.w = t->w; a.end_bonus = t->end_bonus;
+  a.score = r->score; a.qle = r->qle; a.tle = r->tle; a.gtle = r->gtle; a.gscore = r->gscore;
+  a.max_off = r->max_off; a.cells = r->cells;
+  const size_t n = (size_t)t->n;
+  HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
+  HIPCHK(c, c->list.ensure(4 * n + 4), "hipMalloc(list)");
+  HIPCHK(c, c->small.ensure(4 * (3 * kExtBins + 4)), "hipMalloc(work)");
+  ExtWork wk;
+  wk.bin_of = (uint8_t*)c->bin_of.p;
+  wk.list = (int32_t*)c->list.p;
+  wk.counts = (uint32_t*)c->small.p;
+  wk.cursor = wk.counts + kExtBins;
+  wk.fetch = wk.cursor + kExtBins;
+  wk.err = (int32_t*)(wk.fetch + kExtBins);
+  HIPCHK(c, launch_extend(a, wk, c->num_cu, c->stream, nullptr), "extension launch");
+  HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+         "hipMemcpyAsync(err)");
+  c->pending_check = true;
+  c->pending_kind = 1;
+  return BSX_OK;
+}
+
+int bsx_extend_batch(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, bsx_ext_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  int rc = check_scoring(c, sc);
+  if (rc) return rc;
+  if (t->n < 0 || t->n > INT_MAX) return fail(c, BSX_EINVAL, "n must be in [0, 2^31-1]");
+  const size_t n = (size_t)t->n;
+  if (n == 0) return BSX_OK;
+  if (!t->query || !t->q_off || !t->qlen || !t->target || !t->t_off || !t->tlen || !t->h0 || !t->w ||
+      !t->end_bonus)
+    return fail(c, BSX_EINVAL, "task arrays must not be NULL");
+  // extent of the sequence buffers actually referenced
+  uint64_t qext = 0, text = 0;
+  for (size_t k = 0; k < n; ++k) {
+    if (t->qlen[k] > 0 && t->q_off[k] + (uint64_t)t->qlen[k] > qext) qext = t->q_off[k] + t->qlen[k];
+    if (t->tlen[k] > 0 && t->t_off[k] + (uint64_t)t->tlen[k] > text) text = t->t_off[k] + t->tlen[k];
+  }
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, c->h_query.ensure(qext + 1), "hipMalloc(query)");
+  HIPCHK(c, c->h_target.ensure(text + 1), "hipMalloc(target)");
+  HIPCHK(c, c->h_qoff.ensure(8 * n), "hipMalloc(q_off)");
+  HIPCHK(c, c->h_toff.ensure(8 * n), "hipMalloc(t_off)");
+  HIPCHK(c, c->h_i32.ensure(5 * 4 * n), "hipMalloc(task ints)");
+  HIPCHK(c, c->h_res.ensure(8 * 4 * n + 8 * n), "hipMalloc(results)");
+  hipStream_t st = c->stream;
+  uint8_t* dq = (uint8_t*)c->h_query.p;
+  uint8_t* dt = (uint8_t*)c->h_target.p;
+  int32_t* di = (int32_t*)c->h_i32.p;
+  if (qext) HIPCHK(c, hipMemcpyAsync(dq, t->query, qext, hipMemcpyHostToDevice, st), "H2D query");
+  if (text) HIPCHK(c, hipMemcpyAsync(dt, t->target, text, hipMemcpyHostToDevice, st), "H2D target");
+  HIPCHK(c, hipMemcpyAsync(c->h_qoff.p, t->q_off, 8 * n, hipMemcpyHostToDevice, st), "H2D q_off");
+  HIPCHK(c, hipMemcpyAsync(c->h_toff.p, t->t_off, 8 * n, hipMemcpyHostToDevice, st), "H2D t_off");
+  const int32_t* src[5] = {t->qlen, t->tlen, t->h0, t->w, t->end_bonus};
+  for (int k = 0; k < 5; ++k)
+    HIPCHK(c, hipMemcpyAsync(di + k * n, src[k], 4 * n, hipMemcpyHostToDevice, st), "H2D task ints");
+  bsx_ext_tasks dtk{(int64_t)n, dq, (const uint64_t*)c->h_qoff.p, di, dt, (const uint64_t*)c->h_toff.p,
+                    di + n, di + 2 * n, di + 3 * n, di + 4 * n};
+  int32_t* dr = (int32_t*)c->h_res.p;
+  int32_t* outs[6] = {r->score, r->qle, r->tle, r->gtle, r->gscore, r->max_off};
+  bsx_ext_results drs{};
+  int32_t** dslots[6] = {&drs.score, &drs.qle, &drs.tle, &drs.gtle, &drs.gscore, &drs.max_off};
+  for (int k = 0; k < 6; ++k) *dslots[k] = outs[k] ? dr + k * n : nullptr;
+  drs.cells = r->cells ? (int64_t*)(dr + 8 * n) : nullptr;
+  rc = bsx_extend_batch_device(c, sc, &dtk, &drs);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(st), "hipStreamSynchronize");
+  c->pending_check = false;
+  const int rejected = c->err_host[0];
+  std::vector<uint8_t> ok;
+  if (rejected > 0) {  // copy back only the tasks that ran; rejected slots stay untouched
+    ok.resize(n);
+    HIPCHK(c, hipMemcpy(ok.data(), c->bin_of.p, n, hipMemcpyDeviceToHost), "D2H bin_of");
+  }
+  std::vector<int32_t> tmp;
+  std::vector<int64_t> tmp64;
+  for (int k = 0; k < 6; ++k) {
+    if (!outs[k]) continue;
+    if (rejected == 0) {
+      HIPCHK(c, hipMemcpy(outs[k], dr + k * n, 4 * n, hipMemcpyDeviceToHost), "D2H results");
+    } else {
+      tmp.resize(n);
+      HIPCHK(c, hipMemcpy(tmp.data(), dr + k * n, 4 * n, hipMemcpyDeviceToHost), "D2H results");
+      for (size_t q = 0; q < n; ++q)
+        if (ok[q] != 0xff) outs[k][q] = tmp[q];
+    }
+  }
+  if (r->cells) {
+    if (rejected == 0) {
+      HIPCHK(c, hipMemcpy(r->cells, drs.cells, 8 * n, hipMemcpyDeviceToHost), "D2H cells");
+    } else {
+      tmp64.resize(n);
+      HIPCHK(c, hipMemcpy(tmp64.data(), drs.cells, 8 * n, hipMemcpyDeviceToHost), "D2H cells");
+      for (size_t q = 0; q < n; ++q)
+        if (ok[q] != 0xff) r->cells[q] = tmp64[q];
+    }
+  }
+  if (rejected > 0)
+    return fail(c, BSX_EINVAL, "%d task(s) rejected (precondition violated); first is task %d", rejected,
+                c->err_host[1]);
+  return BSX_OK;
+}
+
+void* bsx_dev_alloc(bsx_ctx* c, size_t bytes) {
+  if (!c) return nullptr;
+  (void)hipSetDevice(c->device);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "hipMalloc");
+    return nullptr;
+  }
+  return p;
+}
+
+int bsx_dev_free(bsx_ctx* c, void* p) {
+  if (!c) return BSX_EINVAL;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (p) HIPCHK(c, hipFree(p), "hipFree");
+  return BSX_OK;
+}
+
+int bsx_memcpy(bsx_ctx* c, void* dst, const void* src, size_t bytes, int kind) {
+  if (!c || (bytes && (!dst || !src))) return fail(c, BSX_EINVAL, "bsx_memcpy: NULL argument");
+  hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                  : kind == 3 ? hipMemcpyDeviceToDevice : hipMemcpyDefault;
+  if (kind < 1 || kind > 3) return fail(c, BSX_EINVAL, "bsx_memcpy: kind must be 1, 2 or 3");
+  (void)hipSetDevice(c->device);
+  if (bytes) HIPCHK(c, hipMemcpyAsync(dst, src, bytes, k, c->stream), "hipMemcpyAsync");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return BSX_OK;
+}
+
+int bsx_timer_start(bsx_ctx* c) {
+  if (!c) return BSX_EINVAL;
+  (void)hipSetDevice(c->device);
+  if (!c->ev0) HIPCHK(c, hipEventCreate(&c->ev0), "hipEventCreate");
+  if (!c->ev1) HIPCHK(c, hipEventCreate(&c->ev1), "hipEventCreate");
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+  return BSX_OK;
+}
+
+int bsx_timer_stop(bsx_ctx* c, float* ms) {
+  if (!c || !ms || !c->ev0) return fail(c, BSX_EINVAL, "bsx_timer_stop without bsx_timer_start");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+  HIPCHK(c, hipEventSynchronize(c->ev1), "hipEventSynchronize");
+  HIPCHK(c, hipEventElapsedTime(ms, c->ev0, c->ev1), "hipEventElapsedTime");
+  return BSX_OK;
+}
+
+void bsx_set_global_zslot(bsx_ctx* c, int64_t bytes) {
+  if (c && bytes >= 64) c->zslot = bytes;
+}
+
+int bsx_global_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t,
+                            bsx_glb_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  int rc = check_scoring(c, sc);
+  if (rc) return rc;
+  if (t->n < 0 || t->n > INT_MAX) return fail(c, BSX_EINVAL, "n must be in [0, 2^31-1]");
+  if (t->n > 0 && (!t->query || !t->q_off || !t->qlen || !t->target || !t->t_off || !t->tlen || !t->w))
+    return fail(c, BSX_EINVAL, "task arrays must not be NULL");
+  if (r->cigar && (!r->n_cigar || r->cigar_slot < 1))
+    return fail(c, BSX_EINVAL, "a CIGAR request needs n_cigar and cigar_slot >= 1");
+  (void)hipSetDevice(c->device);
+  GlbArgs a{};
+  a.o_del = sc->o_del; a.e_del = sc->e_del; a.o_ins = sc->o_ins; a.e_ins = sc->e_ins;
+  fill_profile(a, sc);
+  a.n = t->n;
+  a.query = t->query; a.q_off = t->q_off; a.qlen = t->qlen;
+  a.target = t->target; a.t_off = t->t_off; a.tlen = t->tlen; a.w = t->w;
+  a.score = r->score; a.n_cigar = r->n_cigar; a.cigar = r->cigar; a.cigar_slot = r->cigar_slot;
+  const size_t n = (size_t)t->n;
+  HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
+  HIPCHK(c, c->list.ensure(4 * n + 4), "hipMalloc(list)");
+  HIPCHK(c, c->small.ensure(4 * (3 * kExtBins + 4)), "hipMalloc(work)");
+  GlbWork wk;
+  wk.bin_of = (uint8_t*)c->bin_of.p;
+  wk.list = (int32_t*)c->list.p;
+  wk.counts = (uint32_t*)c->small.p;
+  wk.cursor = wk.counts + kExtBins;
+  wk.fetch = wk.cursor + kExtBins;
+  wk.err = (int32_t*)(wk.fetch + kExtBins);
+  wk.zslot = c->zslot;
+  wk.z = nullptr;
+  if (r->cigar) {
+    HIPCHK(c, c->zbuf.ensure((size_t)glb_groups(c->num_cu) * (size_t)c->zslot), "hipMalloc(z)");
+    wk.z = (uint8_t*)c->zbuf.p;
+  }
+  HIPCHK(c, launch_global_impl(a, wk, c->num_cu, c->stream), "global launch");
+  HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+         "hipMemcpyAsync(err)");
+  c->pending_check = true;
+  c->pending_kind = 2;
+  return BSX_OK;
+}
+
+int bsx_global_batch(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t, bsx_glb_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  int rc = check_scoring(c, sc);
+  if (rc) return rc;
+  if (t->n < 0 || t->n > INT_MAX) return fail(c, BSX_EINVAL, "n must be in [0, 2^31-1]");
+  const size_t n = (size_t)t->n;
+  if (n == 0) return BSX_OK;
+  if (!t->query || !t->q_off || !t->qlen || !t->target || !t->t_off || !t->tlen || !t->w)
+    return fail(c, BSX_EINVAL, "task arrays must not be NULL");
+  if (r->cigar && (!r->n_cigar || r->cigar_slot < 1))
+    return fail(c, BSX_EINVAL, "a CIGAR request needs n_cigar and cigar_slot >= 1");
+  uint64_t qext = 0, text = 0;
+  for (size_t k = 0; k < n; ++k) {
+    if (t->qlen[k] > 0 && t->q_off[k] + (uint64_t)t->qlen[k] > qext) qext = t->q_off[k] + t->qlen[k];
+    if (t->tlen[k] > 0 && t->t_off[k] + (uint64_t)t->tlen[k] > text) text = t->t_off[k] + t->tlen[k];
+  }
+  (void)hipSetDevice(c->device);
+  const size_t slot = r->cigar ? (size_t)r->cigar_slot : 0;
+  HIPCHK(c, c->h_query.ensure(qext + 1), "hipMalloc(query)");
+  HIPCHK(c, c->h_target.ensure(text + 1), "hipMalloc(target)");
+  HIPCHK(c, c->h_qoff.ensure(8 * n), "hipMalloc(q_off)");
+  HIPCHK(c, c->h_toff.ensure(8 * n), "hipMalloc(t_off)");
+  HIPCHK(c, c->h_i32.ensure(3 * 4 * n), "hipMalloc(task ints)");
+  HIPCHK(c, c->h_res.ensure(2 * 4 * n + 4 * n * slot), "hipMalloc(results)");
+  hipStream_t st = c->stream;
+  uint8_t* dq = (uint8_t*)c->h_query.p;
+  uint8_t* dt = (uint8_t*)c->h_target.p;
+  int32_t* di = (int32_t*)c->h_i32.p;
+  if (qext) HIPCHK(c, hipMemcpyAsync(dq, t->query, qext, hipMemcpyHostToDevice, st), "H2D query");
+  if (text) HIPCHK(c, hipMemcpyAsync(dt, t->target, text, hipMemcpyHostToDevice, st), "H2D target");
+  HIPCHK(c, hipMemcpyAsync(c->h_qoff.p, t->q_off, 8 * n, hipMemcpyHostToDevice, st), "H2D q_off");
+  HIPCHK(c, hipMemcpyAsync(c->h_toff.p, t->t_off, 8 * n, hipMemcpyHostToDevice, st), "H2D t_off");
+  const int32_t* src[3] = {t->qlen, t->tlen, t->w};
+  for (int k = 0; k < 3; ++k)
+    HIPCHK(c, hipMemcpyAsync(di + k * n, src[k], 4 * n, hipMemcpyHostToDevice, st), "H2D task ints");
+  bsx_glb_tasks dtk{(int64_t)n, dq, (const uint64_t*)c->h_qoff.p, di, dt, (const uint64_t*)c->h_toff.p,
+                    di + n, di + 2 * n};
+  int32_t* dr = (int32_t*)c->h_res.p;
+  bsx_glb_results drs{r->score ? dr : nullptr, r->n_cigar ? dr + n : nullptr,
+                      r->cigar ? (uint32_t*)(dr + 2 * n) : nullptr, r->cigar_slot};
+  rc = bsx_global_batch_device(c, sc, &dtk, &drs);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(st), "hipStreamSynchronize");
+  c->pending_check = false;
+  const int rejected = c->err_host[0], overflow = c->err_host[2];
+  std::vector<uint8_t> ok(n, 0);
+  if (rejected > 0) HIPCHK(c, hipMemcpy(ok.data(), c->bin_of.p, n, hipMemcpyDeviceToHost), "D2H bin_of");
+  std::vector<int32_t> sco(n), ncg(n);
+  std::vector<uint32_t> cg(n * slot);
+  if (r->score) HIPCHK(c, hipMemcpy(sco.data(), drs.score, 4 * n, hipMemcpyDeviceToHost), "D2H score");
+  if (r->n_cigar) HIPCHK(c, hipMemcpy(ncg.data(), drs.n_cigar, 4 * n, hipMemcpyDeviceToHost), "D2H n_cigar");
+  if (r->cigar) HIPCHK(c, hipMemcpy(cg.data(), drs.cigar, 4 * n * slot, hipMemcpyDeviceToHost), "D2H cigar");
+  for (size_t q = 0; q < n; ++q) {
+    if (ok[q] == 0xff) continue;
+    if (r->score) r->score[q] = sco[q];
+    if (r->n_cigar) r->n_cigar[q] = ncg[q];
+    if (r->cigar && ncg[q] > 0) memcpy(r->cigar + q * slot, cg.data() + q * slot, 4 * (size_t)ncg[q]);
+  }
+  if (rejected > 0)
+    return fail(c, BSX_EINVAL, "%d task(s) rejected (precondition violated); first is task %d", rejected,
+                c->err_host[1]);
+  if (overflow > 0) return fail(c, BSX_ERANGE, "%d CIGAR(s) did not fit their slot", overflow);
+  return BSX_OK;
+}
+
+int bsx_ksw_extend2(int qlen, const uint8_t* query, int tlen, const uint8_t* target, int m,
+                    const int8_t* mat, int o_del, int e_del, int o_ins, int e_ins, int w, int end_bonus,
+                    int zdrop, int h0, int* qle, int* tle, int* gtle, int* gscore, int* max_off) {
+  static thread_local bsx_ctx* ctx = nullptr;
+  if (m != 5 || !query || !mat || (tlen > 0 && !target)) return INT32_MIN;
+  if (!ctx) ctx = bsx_create(0);
+  if (!ctx) return INT32_MIN;
+  bsx_scoring sc;
+  memcpy(sc.mat, mat, 25);
+  sc.o_del = o_del; sc.e_del = e_del; sc.o_ins = o_ins; sc.e_ins = e_ins; sc.zdrop = zdrop;
+  const uint64_t zero = 0;
+  static const uint8_t dummy = 0;
+  bsx_ext_tasks t{1, query, &zero, &qlen, tlen > 0 ? target : &dummy, &zero, &tlen, &h0, &w, &end_bonus};
+  int32_t o[6];
+  bsx_ext_results r{&o[0], &o[1], &o[2], &o[3], &o[4], &o[5], nullptr};
+  if (bsx_extend_batch(ctx, &sc, &t, &r) != BSX_OK) return INT32_MIN;
+  if (qle) *qle = o[1];
+  if (tle) *tle = o[2];
+  if (gtle) *gtle = o[3];
+  if (gscore) *gscore = o[4];
+  if (max_off) *max_off = o[5];
+  return o[0];
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Internal declarations shared by the kernels and the C-ABI layer of libbsx.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bsx {
+
+// Extension work is split into bins by query length; each bin runs a kernel
+// instantiation with L lanes per task and C query columns per lane
+// (qlen <= L*C).  Index = bin id.
+constexpr int kExtBins = 8;
+struct BinShape { int lanes, cols; };
+constexpr BinShape kExtShape[kExtBins] = {
+    {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
+constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
+
+// Scores are tracked as (h << 11 | column) keys in 32-bit lanes.
+constexpr int kScoreLimit = 1 << 20;
+
+// Kernel-side view of an extension batch (all pointers are device memory).
+struct ExtArgs {
+  int o_del, e_del, o_ins, e_ins, zdrop, mat_max;
+  // Query profile by query code qc (0..4): bytes k=0..3 of prof_lo[qc] hold
+  // mat[k*5+qc] (target code k), prof_hi[qc] holds mat[20+qc] (target N).
+  uint32_t prof_lo[5];
+  int32_t prof_hi[5];
+  int64_t n;
+  const uint8_t* query;
+  const uint64_t* q_off;
+  const int32_t* qlen;
+  const uint8_t* target;
+  const uint64_t* t_off;
+  const int32_t* tlen;
+  const int32_t* h0;
+  const int32_t* w;
+  const int32_t* end_bonus;
+  int32_t *score, *qle, *tle, *gtle, *gscore, *max_off;
+  int64_t* cells;
+};
+
+// Per-call device workspace for the extension path.
+struct ExtWork {
+  uint8_t* bin_of;     // [n] bin id, 0xff = rejected task
+  int32_t* list;       // [n] task indices grouped by bin
+  uint32_t* counts;    // [kExtBins] tasks per bin
+  uint32_t* cursor;    // [kExtBins] scatter cursors
+  uint32_t* fetch;     // [kExtBins] persistent-kernel fetch counters
+  int32_t* err;        // [0] rejected count, [1] lowest rejected index
+};
+
+hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
+                         const int64_t* host_bin_counts);
+
+struct GlbArgs {
+  int o_del, e_del, o_ins, e_ins;
+  uint32_t prof_lo[5];
+  int32_t prof_hi[5];
+  int64_t n;
+  const uint8_t* query;
+  const uint64_t* q_off;
+  const int32_t* qlen;
+  const uint8_t* target;
+  const uint64_t* t_off;
+  const int32_t* tlen;
+  const int32_t* w;
+  int32_t* score;
+  int32_t* n_cigar;
+  uint32_t* cigar;
+  int64_t cigar_slot;
+};
+
+struct GlbWork {
+  uint8_t* bin_of;
+  int32_t* list;
+  uint32_t* counts;
+  uint32_t* cursor;
+  uint32_t* fetch;
+  int32_t* err;  // [0] rejected, [1] first rejected, [2] CIGAR overflows
+  uint8_t* z;    // per-group direction-byte slots
+  int64_t zslot; // bytes per slot
+};
+
+// Blocks per CU of the global kernel; the scratch holds one slot per group.
+constexpr int kGlbBlocksPerCU = 4;
+int64_t glb_groups(int num_cu);
+hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st);
+
+}  // namespace bsx

@@ -1,0 +1,297 @@
+// Bit-exact batched ksw_global2 (bsbolt/External/BWA/ksw.c:504-606) for gfx950.
+//
+// Same layout as the extension kernel: a group of L lanes owns one task, lane g
+// holds query columns [g*C, g*C+C) of H(i-1,j-1) and E(i,j) in VGPRs, rows are
+// processed in order.  The band is fixed (beg = max(i-w,0), end = min(i+w+1,qlen),
+// ksw.c:530-531).  F along a row is
+//   F(j) = max_{beg<=k<j}(M_k - oe_ins + k*e_ins) - (j-1)*e_ins   (j > beg),
+//   F(beg) = -inf,
+// which equals the reference's sequential recurrence whenever every in-band M
+// is finite -- guaranteed by the precondition |tlen - qlen| <= w together with
+// the length limits checked per task.
+//
+// The direction byte of every in-band cell (bits 0-1: source of H, bit 2: E
+// extended, bit 5: F extended; ksw.c:551-564) is written to the group's slot in
+// a scratch buffer, row-major with n_col = min(qlen, 2w+1) bytes per row, and
+// lane 0 of the group walks it back (ksw.c:588-602), writing the CIGAR into the
+// task's caller-sized slot.
+#include "bsx_internal.h"
+
+namespace bsx {
+
+constexpr int kNegInf = -0x40000000;  // MINUS_INF, ksw.c:490
+constexpr int kGlbMaxT = 1 << 16;
+
+template <int L>
+__device__ __forceinline__ int g_excl_max(int x, int lig, int ident) {
+#pragma unroll
+  for (int d = 1; d < L; d <<= 1) {
+    int y = __shfl_up(x, d, L);
+    if (lig >= d) x = max(x, y);
+  }
+  int e = __shfl_up(x, 1, L);
+  return lig == 0 ? ident : e;
+}
+
+template <int L>
+__device__ __forceinline__ int g_max(int x) {
+#pragma unroll
+  for (int d = L / 2; d >= 1; d >>= 1) x = max(x, __shfl_xor(x, d, L));
+  return x;
+}
+
+template <int L, int C>
+__global__ void __launch_bounds__(256)
+glb_kernel(GlbArgs A, GlbWork W, int bin) {
+  const int lane = threadIdx.x & 63;
+  const int lig = lane & (L - 1);
+  const int base = lig * C;
+  const int group = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+  uint32_t lbase = 0;
+  for (int b = 0; b < bin; ++b) lbase += W.counts[b];
+  const int32_t* __restrict__ list = W.list + lbase;
+  const uint32_t nlist = W.counts[bin];
+  uint8_t* __restrict__ z = W.z + (int64_t)group * W.zslot;
+  const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
+  const int e_del = A.e_del, e_ins = A.e_ins;
+
+  for (;;) {
+    uint32_t idx = 0;
+    if (lig == 0) idx = atomicAdd(W.fetch + bin, 1u);
+    idx = __shfl(idx, 0, L);
+    if (idx >= nlist) break;
+    const int task = list[idx];
+    const int qlen = A.qlen[task], tlen = A.tlen[task], w = A.w[task];
+    const uint64_t qoff = A.q_off[task], toff = A.t_off[task];
+    const int ncol = qlen < 2 * w + 1 ? qlen : 2 * w + 1;
+    const bool want_cigar = A.cigar != nullptr;
+
+    int G[C], E[C];
+    uint32_t plo[C];
+    int phi[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int j = base + c;
+      const int qc = j < qlen ? (int)A.query[qoff + j] : 4;
+      uint32_t lo = A.prof_lo[4];
+      int hi = A.prof_hi[4];
+      if (qc == 0) { lo = A.prof_lo[0]; hi = A.prof_hi[0]; }
+      if (qc == 1) { lo = A.prof_lo[1]; hi = A.prof_hi[1]; }
+      if (qc == 2) { lo = A.prof_lo[2]; hi = A.prof_hi[2]; }
+      if (qc == 3) { lo = A.prof_lo[3]; hi = A.prof_hi[3]; }
+      plo[c] = lo;
+      phi[c] = hi;
+      // row -1 (ksw.c:522-525)
+      G[c] = j == 0 ? 0 : (j <= w ? -(A.o_ins + e_ins * j) : kNegInf);
+      E[c] = kNegInf;
+    }
+    int hq = qlen <= w ? -(A.o_ins + e_ins * qlen) : kNegInf;  // eh[qlen].h
+
+    uint32_t tch = 0;
+    int tcb = -(1 << 30);
+    for (int i = 0; i < tlen; ++i) {
+      if (i - tcb >= 4 * L) {
+        tcb = i;
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int r = i + 4 * lig + k;
+          v |= (r < tlen ? (uint32_t)A.target[toff + r] : 0u) << (8 * k);
+        }
+        tch = v;
+      }
+      const int tr = i - tcb;
+      const int t = (int)((__shfl(tch, tr >> 2, L) >> (8 * (tr & 3))) & 0xffu);
+      const int beg = i > w ? i - w : 0;
+      const int end = i + w + 1 < qlen ? i + w + 1 : qlen;
+      const int h1p = beg == 0 ? -(A.o_del + e_del * (i + 1)) : kNegInf;
+
+      int M[C], ex[C];
+      int run = INT32_MIN;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int j = base + c;
+        const bool inb = (j >= beg) & (j < end);
+        const int s = t < 4 ? (int)((int32_t)(plo[c] << (24 - 8 * t)) >> 24) : phi[c];
+        const int mm = G[c] + s;
+        M[c] = mm;
+        const int key = inb ? mm - oe_ins + j * e_ins : INT32_MIN;
+        ex[c] = run;
+        run = max(run, key);
+      }
+      const int lex = g_excl_max<L>(run, lig, INT32_MIN);
+      int h[C];
+      int h1c = INT32_MIN;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int j = base + c;
+        const bool inb = (j >= beg) & (j < end);
+        const int x = max(ex[c], lex);
+        const int f = j == beg ? kNegInf : x - (j - 1) * e_ins;
+        const int mm = M[c], e = E[c];
+        int d = mm >= e ? 0 : 1;
+        int hh = mm >= e ? mm : e;
+        if (!(hh >= f)) { d = 2; hh = f; }
+        h[c] = hh;
+        const int od = mm - oe_del, ee = e - e_del;
+        if (ee > od) d |= 1 << 2;
+        const int ne = ee > od ? ee : od;
+        const int oi = mm - oe_ins, ff = f - e_ins;
+        if (ff > oi) d |= 2 << 4;
+        if (inb) {
+          E[c] = ne;
+          if (want_cigar) z[(int64_t)i * ncol + (j - beg)] = (uint8_t)d;
+          if (j == end - 1) h1c = hh;
+        }
+      }
+      const int hprev = __shfl_up(h[C - 1], 1, L);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int j = base + c;
+        const int left = c == 0 ? hprev : h[c - 1];
+        if ((j >= beg) & (j <= end) & (j < qlen)) G[c] = j == beg ? h1p : left;
+        if (j == end && j < qlen) E[c] = kNegInf;
+      }
+      h1c = g_max<L>(h1c);
+      const int h1 = end > beg ? h1c : h1p;
+      if (end == qlen) hq = h1;  // eh[end] = {h1, -inf} with end == qlen
+    }
+    if (lig == 0) {
+      if (A.score) A.score[task] = hq;
+    }
+    if (want_cigar) {
+      // The slot is written by every lane of this group and read back by its
+      // lane 0 below; all of them are lanes of one wave, so ordering the
+      // stores before the loads within the wave is all that is needed.
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      if (lig == 0) {
+        uint32_t* cg = A.cigar + (int64_t)task * A.cigar_slot;
+        const int64_t cap = A.cigar_slot;
+        int n = 0, state = 0;
+        bool over = false;
+        int i = tlen - 1;
+        int k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
+        auto push = [&](int op, int len) {
+          if (n > 0 && (int)(cg[n - 1] & 0xf) == op) {
+            cg[n - 1] += (uint32_t)len << 4;
+          } else if (n < cap) {
+            cg[n++] = (uint32_t)len << 4 | (uint32_t)op;
+          } else {
+            over = true;
+          }
+        };
+        while (i >= 0 && k >= 0 && !over) {
+          const int lo = i > w ? i - w : 0;
+          if (k < lo || k - lo >= ncol) {  // cannot happen under the preconditions; never read outside the slot
+            over = true;
+            break;
+          }
+          state = z[(int64_t)i * ncol + (k - lo)] >> (state << 1) & 3;
+          if (state == 0) { push(0, 1); --i; --k; }
+          else if (state == 1) { push(2, 1); --i; }
+          else { push(1, 1); --k; }
+        }
+        if (!over && i >= 0) push(2, i + 1);
+        if (!over && k >= 0) push(1, k + 1);
+        if (over) {
+          A.n_cigar[task] = -1;
+          atomicAdd(&W.err[2], 1);
+        } else {
+          for (int a = 0, b = n - 1; a < b; ++a, --b) { const uint32_t tmp = cg[a]; cg[a] = cg[b]; cg[b] = tmp; }
+          A.n_cigar[task] = n;
+        }
+      }
+    } else if (A.n_cigar && lig == 0) {
+      A.n_cigar[task] = 0;
+    }
+  }
+}
+
+__device__ __forceinline__ int glb_bin_of(int qlen) {
+#pragma unroll
+  for (int b = 0; b < kExtBins; ++b)
+    if (qlen <= bin_limit(b)) return b;
+  return -1;
+}
+
+__global__ void __launch_bounds__(256)
+glb_bin_count_kernel(GlbArgs A, GlbWork W) {
+  __shared__ uint32_t hist[kExtBins];
+  if (threadIdx.x < kExtBins) hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int ql = A.qlen[t], tl = A.tlen[t], w = A.w[t];
+    int b = glb_bin_of(ql);
+    const int diff = tl > ql ? tl - ql : ql - tl;
+    const int64_t ncol = ql < 2 * (int64_t)w + 1 ? ql : 2 * (int64_t)w + 1;
+    const bool ok = ql >= 1 && tl >= 1 && tl <= kGlbMaxT && w >= 0 && b >= 0 && diff <= w &&
+                    (A.cigar == nullptr || ncol * tl <= W.zslot);
+    if (!ok) {
+      b = 0xff;
+      atomicAdd(&W.err[0], 1);
+      atomicMin(&W.err[1], (int)t);
+    } else {
+      atomicAdd(&hist[b], 1u);
+    }
+    W.bin_of[t] = (uint8_t)b;
+  }
+  __syncthreads();
+  if (threadIdx.x < kExtBins && hist[threadIdx.x]) atomicAdd(&W.counts[threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256)
+glb_bin_scatter_kernel(int64_t n, GlbWork W) {
+  __shared__ uint32_t off[kExtBins];
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int b = 0; b < kExtBins; ++b) { off[b] = s; s += W.counts[b]; }
+  }
+  __syncthreads();
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int b = W.bin_of[t];
+    if (b == 0xff) continue;
+    const uint32_t p = atomicAdd(&W.cursor[b], 1u);
+    W.list[off[b] + p] = (int32_t)t;
+  }
+}
+
+__global__ void glb_reset_kernel(GlbWork W) {
+  const int k = threadIdx.x;
+  if (k < kExtBins) { W.counts[k] = 0; W.cursor[k] = 0; W.fetch[k] = 0; }
+  if (k == 0) { W.err[0] = 0; W.err[1] = 0x7fffffff; W.err[2] = 0; }
+}
+
+
+int64_t glb_groups(int num_cu) {
+  // the largest group count over the bins is with 16 lanes per task
+  return (int64_t)num_cu * kGlbBlocksPerCU * (256 / 16);
+}
+
+template <int B>
+static void glb_launch_bin(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st) {
+  constexpr int L = kExtShape[B].lanes, C = kExtShape[B].cols;
+  hipLaunchKernelGGL((glb_kernel<L, C>), dim3((unsigned)(num_cu * kGlbBlocksPerCU)), dim3(256), 0, st, a, w,
+                     B);
+}
+
+hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st) {
+  hipLaunchKernelGGL(glb_reset_kernel, dim3(1), dim3(64), 0, st, w);
+  if (a.n <= 0) return hipGetLastError();
+  int64_t blocks = (a.n + 255) / 256;
+  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  hipLaunchKernelGGL(glb_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, w);
+  hipLaunchKernelGGL(glb_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a.n, w);
+  glb_launch_bin<7>(a, w, num_cu, st);
+  glb_launch_bin<6>(a, w, num_cu, st);
+  glb_launch_bin<5>(a, w, num_cu, st);
+  glb_launch_bin<4>(a, w, num_cu, st);
+  glb_launch_bin<3>(a, w, num_cu, st);
+  glb_launch_bin<2>(a, w, num_cu, st);
+  glb_launch_bin<1>(a, w, num_cu, st);
+  glb_launch_bin<0>(a, w, num_cu, st);
+  return hipGetLastError();
+}
+
+}  // namespace bsx

@@ -1,0 +1,242 @@
+"""Workload generator: synthetic bisulfite genomes and reads, turned into the
+ksw_extend2 / ksw_global2 task batches that BSBolt's aligner would issue.
+
+Python front end of libbsxwl.so (bsbolt_amd/csrc/workload.cpp).  The
+extension rounds of mem_chain2aln (bsbolt/External/BWA/bwamem.c:636-790) need
+the results of earlier rounds (right-flank h0 = left score, w<<1 retries, seed
+containment), so `build_ext_batch` takes an `extend` callable: in the product it
+is the GPU engine (bsx.Engine.extend).
+
+Named workloads (sizes in bases; seeds fixed so every run sees the same batch):
+  cfg0  2 Mb genome,  20k single-end 100 bp reads
+  cfg1  50 Mb genome, 1M paired-end 2x150 pairs, insert ~ N(350, 50)
+  cfg2  training set for the band predictor: cfg1 recipe, disjoint seeds
+  cfg3  100 Mb genome, 500k single-end 250 bp reads, 1% subs, 0.5% indels (geometric, <= 30)
+  cfg4  100 Mb genome, RRBS: 2M single-end reads from MspI fragments, 40-100 bp, 20% duplicates
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .tasks import ExtTasks, GlbTasks, RESULT_FIELDS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WL_PATH = os.path.join(HERE, "libbsxwl.so")
+
+# BSBolt Align defaults (bsbolt/Utils/Parser.py:40-106 -> bwa mem argv,
+# bsbolt/Utils/Launcher.py:77-108): -A1 -B4 -O6,6 -E1,1 -L30,30 -w100 -d100 -k19
+DEFAULT_OPT = dict(a=1, b=4, o_del=6, e_del=1, o_ins=6, e_ins=1, pen_clip5=30, pen_clip3=30, w=100,
+                   zdrop=100, min_seed_len=19)
+
+
+@dataclass
+class SimConfig:
+    genome_len: int
+    genome_seed: int
+    n_frag: int
+    read_seed: int
+    read_len: int = 100
+    paired: bool = False
+    insert_mean: float = 350.0
+    insert_sd: float = 50.0
+    p_cpg: float = 0.8
+    p_ch: float = 0.01
+    sub: float = 0.005
+    indel: float = 0.0005
+    geometric_indels: bool = False
+    trim_min: int = 0
+    rrbs: bool = False
+    dup: float = 0.0
+    p_shadow: float = 0.7        # reads that also extend one shadow chain
+    p_shadow_len: float = 0.25   # shadow seed length = 19 + Geometric(p) - 1
+    p_at: float = 0.295          # 41% GC
+    cpg_frac: float = 0.01       # CpG share of dinucleotides
+    opt: dict = field(default_factory=lambda: dict(DEFAULT_OPT))
+
+    def scaled(self, frac):
+        """Same recipe with a fraction of the reads (and genome >= 1 Mb)."""
+        import dataclasses
+        return dataclasses.replace(self, n_frag=max(1, int(self.n_frag * frac)),
+                                   genome_len=max(1_000_000, int(self.genome_len * min(1.0, frac * 4))))
+
+
+CONFIGS = {
+    "cfg0": SimConfig(2_000_000, 0, 20_000, 100, read_len=100),
+    "cfg1": SimConfig(50_000_000, 1, 1_000_000, 101, read_len=150, paired=True),
+    "cfg2": SimConfig(50_000_000, 2, 500_000, 102, read_len=150, paired=True),
+    "cfg3": SimConfig(100_000_000, 3, 500_000, 103, read_len=250, sub=0.01, indel=0.005,
+                      geometric_indels=True),
+    "cfg4": SimConfig(100_000_000, 4, 2_000_000, 104, read_len=100, trim_min=40, rrbs=True, dup=0.2,
+                      p_cpg=0.7),
+}
+
+_wl = None
+_P = C.c_void_p
+
+
+def wl():
+    global _wl
+    if _wl is None:
+        if not os.path.exists(WL_PATH):
+            raise RuntimeError(f"libbsxwl.so not built ({WL_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(WL_PATH)
+        L.wl_genome.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, _P]
+        L.wl_bs_pac.argtypes = [_P, C.c_int64, _P]
+        L.wl_create.restype = _P
+        L.wl_create.argtypes = [_P, _P, C.c_int64]
+        L.wl_destroy.argtypes = [_P]
+        L.wl_simulate.restype = C.c_int64
+        L.wl_simulate.argtypes = [_P, _P, C.c_uint64, C.c_int64, _P, _P, _P, _P, _P, _P, C.c_int64, _P]
+        L.wl_read_len.restype = C.c_int32
+        L.wl_read_len.argtypes = [_P, C.c_int64]
+        L.wl_read_seq.argtypes = [_P, C.c_int64, _P]
+        L.wl_chains_init.restype = C.c_int64
+        L.wl_chains_init.argtypes = [_P, C.c_int64, _P, _P, _P, _P, _P]
+        L.wl_round_emit.restype = C.c_int64
+        L.wl_round_emit.argtypes = [_P]
+        L.wl_round_sizes.argtypes = [_P, _P, _P]
+        L.wl_round_tasks.argtypes = [_P] + [_P] * 11
+        L.wl_round_consume.argtypes = [_P] + [_P] * 6
+        L.wl_n_calls.restype = C.c_int64
+        L.wl_n_calls.argtypes = [_P]
+        L.wl_regions.restype = C.c_int64
+        L.wl_regions.argtypes = [_P, C.c_int64, _P, C.c_int64]
+        L.wl_n_reads.restype = C.c_int64
+        L.wl_n_reads.argtypes = [_P]
+        L.wl_get_seq.restype = C.c_int64
+        L.wl_get_seq.argtypes = [_P, C.c_int64, C.c_int64, _P]
+        _wl = L
+    return _wl
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def make_genome(length, seed, p_at=0.295, cpg=0.01):
+    g = np.zeros(length, np.uint8)
+    wl().wl_genome(seed, length, p_at, cpg, _p(g))
+    return g
+
+
+def bs_pac(genome):
+    """Forward-only 2-bit pac of [genome C->T] ++ [genome G->A]; l_pac = 2*len."""
+    pac = np.zeros((2 * len(genome) + 3) // 4, np.uint8)
+    wl().wl_bs_pac(_p(genome), len(genome), _p(pac))
+    return pac
+
+
+class Workload:
+    """Genome + bisulfite index + simulated reads + their seeds."""
+
+    def __init__(self, cfg: SimConfig):
+        self.cfg = cfg
+        L = wl()
+        self.genome = make_genome(cfg.genome_len, cfg.genome_seed, cfg.p_at, cfg.cpg_frac)
+        self.pac = bs_pac(self.genome)
+        o = cfg.opt
+        self.opt = np.array([o[k] for k in ("a", "b", "o_del", "e_del", "o_ins", "e_ins", "pen_clip5",
+                                            "pen_clip3", "w", "zdrop", "min_seed_len")], np.int32)
+        self.ctx = L.wl_create(_p(self.opt), _p(self.pac), cfg.genome_len)
+        simcfg = np.array([cfg.read_len, int(cfg.paired), cfg.insert_mean, cfg.insert_sd, cfg.p_cpg, cfg.p_ch,
+                           cfg.sub, cfg.indel, int(cfg.geometric_indels), cfg.trim_min, int(cfg.rrbs), cfg.dup, cfg.p_shadow, cfg.p_shadow_len],
+                          np.float64)
+        cap = cfg.n_frag * (2 if cfg.paired else 1) * (cfg.read_len // max(1, o["min_seed_len"]) + 2)
+        self.seed_read = np.zeros(cap, np.int32)
+        self.seed_chain = np.zeros(cap, np.int32)
+        self.seed_rbeg = np.zeros(cap, np.int64)
+        self.seed_qbeg = np.zeros(cap, np.int32)
+        self.seed_len = np.zeros(cap, np.int32)
+        ns = C.c_int64(0)
+        self.n_reads = L.wl_simulate(self.ctx, _p(self.genome), cfg.read_seed, cfg.n_frag, _p(simcfg),
+                                     _p(self.seed_read), _p(self.seed_chain), _p(self.seed_rbeg), _p(self.seed_qbeg),
+                                     _p(self.seed_len), cap, C.byref(ns))
+        if self.n_reads < 0:
+            raise ValueError("simulation failed (no RRBS fragments?)")
+        n = ns.value
+        self.seed_read, self.seed_rbeg = self.seed_read[:n], self.seed_rbeg[:n]
+        self.seed_chain = self.seed_chain[:n]
+        self.seed_qbeg, self.seed_len = self.seed_qbeg[:n], self.seed_len[:n]
+        self.l_pac = 2 * cfg.genome_len
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            wl().wl_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def read(self, i):
+        n = wl().wl_read_len(self.ctx, i)
+        out = np.zeros(n, np.uint8)
+        wl().wl_read_seq(self.ctx, i, _p(out))
+        return out
+
+    def get_seq(self, beg, end):
+        out = np.zeros(abs(end - beg) + 1, np.uint8)
+        n = wl().wl_get_seq(self.ctx, beg, end, _p(out))
+        return out[:n]
+
+    def regions(self, i, cap=64):
+        out = np.zeros((cap, 9), np.int64)
+        n = wl().wl_regions(self.ctx, i, _p(out), cap)
+        return out[:min(n, cap)]
+
+    def build_ext_batch(self, extend, max_rounds=64, on_round=None):
+        """Runs mem_chain2aln over every read in rounds; `extend(ExtTasks) ->
+        dict of result arrays`.  Returns (all tasks issued as one ExtTasks,
+        their results, per-task side 0=left/1=right, per-task round)."""
+        L = wl()
+        L.wl_chains_init(self.ctx, len(self.seed_read), _p(self.seed_read), _p(self.seed_chain), _p(self.seed_rbeg),
+                         _p(self.seed_qbeg), _p(self.seed_len))
+        parts, results, sides, rounds = [], [], [], []
+        for r in range(max_rounds):
+            n = L.wl_round_emit(self.ctx)
+            if n == 0:
+                break
+            qt, tt = C.c_int64(0), C.c_int64(0)
+            L.wl_round_sizes(self.ctx, C.byref(qt), C.byref(tt))
+            q = np.zeros(qt.value + 1, np.uint8)
+            t = np.zeros(tt.value + 1, np.uint8)
+            qo = np.zeros(n, np.uint64); to = np.zeros(n, np.uint64)
+            ql, tl, h0, w, eb, cid, side = [np.zeros(n, np.int32) for _ in range(7)]
+            L.wl_round_tasks(self.ctx, _p(q), _p(qo), _p(ql), _p(t), _p(to), _p(tl), _p(h0), _p(w), _p(eb),
+                             _p(cid), _p(side))
+            tasks = ExtTasks(q, qo, ql, t, to, tl, h0, w, eb)
+            res = extend(tasks)
+            L.wl_round_consume(self.ctx, *[_p(np.ascontiguousarray(res[k], np.int32)) for k in RESULT_FIELDS])
+            parts.append(tasks); results.append({k: np.asarray(res[k]) for k in RESULT_FIELDS})
+            sides.append(side); rounds.append(np.full(n, r, np.int32))
+            if on_round:
+                on_round(r, n)
+        else:
+            raise RuntimeError("mem_chain2aln rounds did not converge")
+        batch = concat_ext(parts)
+        res = {k: np.concatenate([x[k] for x in results]) if results else np.zeros(0, np.int32)
+               for k in RESULT_FIELDS}
+        return batch, res, (np.concatenate(sides) if sides else np.zeros(0, np.int32)), \
+            (np.concatenate(rounds) if rounds else np.zeros(0, np.int32))
+
+
+def concat_ext(parts):
+    if not parts:
+        z = np.zeros(0, np.int32)
+        return ExtTasks(np.zeros(0, np.uint8), np.zeros(0, np.uint64), z, np.zeros(0, np.uint8),
+                        np.zeros(0, np.uint64), z, z, z, z)
+    qbase = np.cumsum([0] + [len(p.query) for p in parts[:-1]], dtype=np.uint64)
+    tbase = np.cumsum([0] + [len(p.target) for p in parts[:-1]], dtype=np.uint64)
+    return ExtTasks(np.concatenate([p.query for p in parts]),
+                    np.concatenate([p.q_off + qb for p, qb in zip(parts, qbase)]),
+                    np.concatenate([p.qlen for p in parts]),
+                    np.concatenate([p.target for p in parts]),
+                    np.concatenate([p.t_off + tb for p, tb in zip(parts, tbase)]),
+                    np.concatenate([p.tlen for p in parts]),
+                    np.concatenate([p.h0 for p in parts]),
+                    np.concatenate([p.w for p in parts]),
+                    np.concatenate([p.end_bonus for p in parts]))

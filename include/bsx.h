@@ -1,0 +1,162 @@
+/*
+ * bsx.h -- C ABI of libbsx.so, the MI355X (gfx950) seed-extend engine for
+ * BSBolt's bisulfite aligner (forked BWA-MEM 0.7.17 in bsbolt/External/BWA).
+ *
+ * The library replaces two scalar routines of the reference with batched,
+ * bit-exact HIP kernels:
+ *   ksw_extend2  ksw.h:108, ksw.c:380-479   -> bsx_extend_batch / bsx_extend_batch_device
+ *   ksw_global2  ksw.h:84,  ksw.c:504-606   -> bsx_global_batch / bsx_global_batch_device
+ * Their callers in the reference are mem_chain2aln (bwamem.c:735,763) and
+ * bwa_gen_cigar2 (bwa.c:248).  Each task of a batch carries exactly the
+ * arguments one reference call takes; each result carries exactly its outputs.
+ *
+ * Conventions
+ *  - Sequences are 1 byte per base, codes 0..4 (A C G T N, bntseq.c:48-65),
+ *    concatenated; task t uses query[q_off[t] .. q_off[t]+qlen[t]) and
+ *    target[t_off[t] .. t_off[t]+tlen[t]).
+ *  - The scoring matrix is 5x5, indexed mat[target*5 + query] (ksw.c:390-393),
+ *    as built by bwa_fill_scmat (bwa.c:169-178).
+ *  - Every entry point returns BSX_OK or a negative BSX_E* code and never
+ *    aborts.  A task that violates a precondition is skipped (its result slots
+ *    are left untouched), the others are computed, and the call returns
+ *    BSX_EINVAL with the first offending task named by bsx_last_error().
+ *  - A context is bound to one device and one stream; calls on distinct
+ *    contexts are thread-safe, calls on one context are not.
+ */
+#ifndef BSX_H
+#define BSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSX_OK        0
+#define BSX_EINVAL   (-1) /* a task or argument violates a precondition        */
+#define BSX_EDEVICE  (-2) /* HIP runtime error (text in bsx_last_error)         */
+#define BSX_ENOMEM   (-3) /* device or host allocation failed                  */
+#define BSX_ERANGE   (-4) /* a CIGAR did not fit its caller-sized slot          */
+
+/* Largest query (flank) length the extension kernels take; longer -> BSX_EINVAL. */
+#define BSX_MAX_QLEN 2048
+
+typedef struct bsx_ctx bsx_ctx;
+
+/* Scalar parameters shared by every task of a batch.  Mirrors the arguments
+ * m=5, mat, o_del, e_del, o_ins, e_ins, zdrop of ksw_extend2 (ksw.h:108). */
+typedef struct {
+	int8_t  mat[25];
+	int32_t o_del, e_del, o_ins, e_ins; /* e_del, e_ins >= 1; o_del, o_ins >= 0 */
+	int32_t zdrop;                      /* <= 0 disables z-drop (ksw.c:458)    */
+} bsx_scoring;
+
+/* One ksw_extend2 call per task.  Preconditions per task (ksw.c:385,388,395):
+ * 1 <= qlen <= BSX_MAX_QLEN, tlen >= 0, h0 > 0, w >= 0,
+ * h0 + qlen*max(mat) < 2^20 (scores are tracked exactly in 32-bit lanes). */
+typedef struct {
+	int64_t n;
+	const uint8_t  *query;
+	const uint64_t *q_off;
+	const int32_t  *qlen;
+	const uint8_t  *target;
+	const uint64_t *t_off;
+	const int32_t  *tlen;
+	const int32_t  *h0;
+	const int32_t  *w;
+	const int32_t  *end_bonus; /* pen_clip5 for left flanks, pen_clip3 for right */
+} bsx_ext_tasks;
+
+/* Outputs of ksw_extend2 (ksw.c:473-478); any pointer may be NULL. */
+typedef struct {
+	int32_t *score;   /* return value: best local score (>= h0)           */
+	int32_t *qle;     /* *_qle                                            */
+	int32_t *tle;     /* *_tle                                            */
+	int32_t *gtle;    /* *_gtle                                           */
+	int32_t *gscore;  /* *_gscore (-1 if the query end was never reached) */
+	int32_t *max_off; /* *_max_off                                        */
+	int64_t *cells;   /* DP cells evaluated: sum over rows of (end - beg) */
+} bsx_ext_results;
+
+/* One ksw_global2 call per task.  Preconditions: qlen >= 1, tlen >= 1,
+ * |tlen - qlen| <= w (the reference reads uninitialised memory otherwise). */
+typedef struct {
+	int64_t n;
+	const uint8_t  *query;
+	const uint64_t *q_off;
+	const int32_t  *qlen;
+	const uint8_t  *target;
+	const uint64_t *t_off;
+	const int32_t  *tlen;
+	const int32_t  *w;
+} bsx_glb_tasks;
+
+/* Outputs of ksw_global2: score, *n_cigar and the BAM CIGAR (len<<4|op,
+ * op 0=M 1=I 2=D), written to cigar[t*cigar_slot ...].  A CIGAR longer than
+ * cigar_slot sets n_cigar[t] = -1 and the call returns BSX_ERANGE.
+ * cigar == NULL asks for scores only (ksw.c:510-513). */
+typedef struct {
+	int32_t  *score;
+	int32_t  *n_cigar;
+	uint32_t *cigar;
+	int64_t   cigar_slot;
+} bsx_glb_results;
+
+/* ---- context ---- */
+bsx_ctx    *bsx_create(int device);               /* NULL on failure             */
+void        bsx_destroy(bsx_ctx *ctx);
+const char *bsx_last_error(const bsx_ctx *ctx);   /* never NULL                  */
+int         bsx_set_stream(bsx_ctx *ctx, void *hip_stream); /* NULL = ctx's own  */
+int         bsx_sync(bsx_ctx *ctx);               /* wait; report deferred errors */
+const char *bsx_version(void);
+
+/* Default scoring of BSBolt Align (Parser.py:47-81: -A1 -B4 -O6,6 -E1,1 -d100). */
+void bsx_default_scoring(bsx_scoring *sc);
+
+/* ---- extension (ksw_extend2) ----
+ * Host-memory form: copies tasks in, runs, copies results out; synchronous. */
+int bsx_extend_batch(bsx_ctx *ctx, const bsx_scoring *sc,
+                     const bsx_ext_tasks *tasks, bsx_ext_results *res);
+/* Device-memory form: every pointer in tasks/res is device memory; the work is
+ * queued on the context's stream and the call returns without waiting.
+ * Precondition violations are reported by the next bsx_sync(). */
+int bsx_extend_batch_device(bsx_ctx *ctx, const bsx_scoring *sc,
+                            const bsx_ext_tasks *tasks, bsx_ext_results *res);
+
+/* ---- global alignment + CIGAR (ksw_global2) ---- */
+int bsx_global_batch(bsx_ctx *ctx, const bsx_scoring *sc,
+                     const bsx_glb_tasks *tasks, bsx_glb_results *res);
+int bsx_global_batch_device(bsx_ctx *ctx, const bsx_scoring *sc,
+                            const bsx_glb_tasks *tasks, bsx_glb_results *res);
+/* Bytes of direction scratch per in-flight task (default 65536): a CIGAR task
+ * needs min(qlen, 2w+1) * tlen bytes; larger ones are rejected (BSX_EINVAL).
+ * The context holds (#CU * 64) such slots of device memory. */
+void bsx_set_global_zslot(bsx_ctx *ctx, int64_t bytes);
+
+/* ---- device memory and timing on the context's device/stream ----
+ * Helpers for callers that hold their batches in device memory without a
+ * HIP runtime of their own (e.g. Python through ctypes).  No counterpart in
+ * the reference, which has no device. */
+void *bsx_dev_alloc(bsx_ctx *ctx, size_t bytes);             /* NULL on failure   */
+int   bsx_dev_free(bsx_ctx *ctx, void *ptr);
+/* kind: 1 host->device, 2 device->host, 3 device->device; waits for completion */
+int   bsx_memcpy(bsx_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+/* Records a start / stop event on the context's stream; bsx_timer_stop waits
+ * for the stop event and returns the elapsed milliseconds in *ms. */
+int   bsx_timer_start(bsx_ctx *ctx);
+int   bsx_timer_stop(bsx_ctx *ctx, float *ms);
+
+/* Scalar drop-in with the exact signature of ksw_extend2 (ksw.h:108), run as
+ * a one-task batch on the calling thread's default context (device 0).  m must
+ * be 5.  On a precondition violation or device error it returns INT32_MIN and
+ * leaves the outputs untouched (the reference asserts instead, ksw.c:385). */
+int bsx_ksw_extend2(int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                    int m, const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins,
+                    int w, int end_bonus, int zdrop, int h0,
+                    int *qle, int *tle, int *gtle, int *gscore, int *max_off);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSX_H */

@@ -1,0 +1,79 @@
+"""GPU parity of the global/CIGAR kernel against the reference's golden outputs
+and the CPU oracle (scores and CIGARs bit-exact)."""
+import numpy as np
+import pytest
+
+from conftest import GLB_GOLDEN, golden_glb
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def scoring_of(sc):
+    from bsbolt_amd.bsx import Scoring
+    return Scoring.make(mat=sc["mat"], o_del=sc["o_del"], e_del=sc["e_del"], o_ins=sc["o_ins"],
+                        e_ins=sc["e_ins"])
+
+
+@pytest.mark.parametrize("name", GLB_GOLDEN)
+def test_global_matches_reference_golden(engine, name):
+    t, sc, rs, rn, rc = golden_glb(name)
+    score, ncig, cig = engine.global_(t, scoring_of(sc), slot=rc.shape[1])
+    assert np.array_equal(score, rs)
+    assert np.array_equal(ncig, rn)
+    for i in range(len(t)):
+        assert np.array_equal(cig[i, :ncig[i]], rc[i, :rn[i]]), f"task {i}"
+
+
+def test_global_score_only(engine):
+    t, sc, rs, rn, rc = golden_glb("glb_random_bsbolt.npz")
+    score, ncig, cig = engine.global_(t, scoring_of(sc), cigar=False)
+    assert np.array_equal(score, rs)
+
+
+def test_global_random_vs_oracle(engine):
+    from bsbolt_amd.tasks import GlbTasks
+    rng = np.random.default_rng(44)
+    pairs, ws = [], []
+    for _ in range(20000):
+        ql = int(rng.integers(1, 260))
+        q = rng.integers(0, 5, ql).astype(np.uint8)
+        t = list(np.where(rng.random(ql) < 0.05, rng.integers(0, 4, ql), q % 4))
+        for _ in range(int(rng.integers(0, 4))):
+            p = int(rng.integers(0, len(t) + 1)); L = int(rng.integers(1, 12))
+            if rng.random() < 0.5:
+                t[p:p] = list(rng.integers(0, 4, L))
+            else:
+                del t[p:p + L]
+        t = np.array(t or [1], np.uint8)
+        pairs.append((q, t)); ws.append(O.global_band(ql, len(t), 100) + int(rng.integers(0, 3)))
+    tk = GlbTasks.from_pairs(pairs, ws)
+    slot = int((tk.qlen + tk.tlen).max())
+    score, ncig, cig = engine.global_(tk, slot=slot)
+    os_, on, oc, _ = O.global2_batch(tk, slot)
+    assert np.array_equal(score, os_)
+    assert np.array_equal(ncig, on)
+    for i in range(len(tk)):
+        assert np.array_equal(cig[i, :ncig[i]], oc[i, :on[i]]), f"task {i}"
+
+
+def test_global_rejects_out_of_band_tasks(engine):
+    from bsbolt_amd.bsx import BsxError, BSX_EINVAL
+    from bsbolt_amd.tasks import GlbTasks
+    q = np.array([0, 1, 2, 3] * 5, np.uint8)
+    pairs = [(q, q), (q, np.concatenate([q, q])), (q, q[:18])]
+    tk = GlbTasks.from_pairs(pairs, [5, 5, 5])   # task 1: |tlen-qlen| = 20 > w
+    (score, ncig, cig), rc = engine.global_(tk, raise_on_error=False)
+    assert rc == BSX_EINVAL
+    assert score[0] == 20 and ncig[0] == 1 and cig[0, 0] == (20 << 4)
+    assert score[1] == 0  # untouched
+
+
+def test_global_cigar_slot_overflow(engine):
+    from bsbolt_amd.bsx import BSX_ERANGE
+    from bsbolt_amd.tasks import GlbTasks
+    q = np.array([0, 1, 2, 3] * 10, np.uint8)
+    t = np.concatenate([q[:10], q[12:25], [0, 0, 0], q[25:]]).astype(np.uint8)
+    tk = GlbTasks.from_pairs([(q, t)], 10)
+    (score, ncig, cig), rc = engine.global_(tk, slot=2, raise_on_error=False)
+    assert rc == BSX_ERANGE and ncig[0] == -1
