@@ -84,9 +84,10 @@ ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t
 
   for (;;) {
     if (need && !done) {
-      uint32_t idx = 0;
-      if (lig == 0) idx = atomicAdd(fetch, 1u);
-      idx = __shfl(idx, 0, L);
+      // Branch-free fetch: every lane of the group issues the atomic (only lane
+      // 0 adds 1), so no lane-divergent region precedes the broadcast.
+      const uint32_t mine = atomicAdd(fetch, lig == 0 ? 1u : 0u);
+      const uint32_t idx = __shfl(mine, 0, L);
       if (idx >= nlist) {
         done = true;
       } else {

@@ -56,9 +56,10 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
   const int e_del = A.e_del, e_ins = A.e_ins;
 
   for (;;) {
-    uint32_t idx = 0;
-    if (lig == 0) idx = atomicAdd(W.fetch + bin, 1u);
-    idx = __shfl(idx, 0, L);
+    // Branch-free fetch: every lane issues the atomic (only lane 0 of the group
+    // adds 1), so no lane-divergent region precedes the broadcast.
+    const uint32_t mine = atomicAdd(W.fetch + bin, lig == 0 ? 1u : 0u);
+    const uint32_t idx = __shfl(mine, 0, L);
     if (idx >= nlist) break;
     const int task = list[idx];
     const int qlen = A.qlen[task], tlen = A.tlen[task], w = A.w[task];
@@ -127,7 +128,9 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
         const int j = base + c;
         const bool inb = (j >= beg) & (j < end);
         const int x = max(ex[c], lex);
-        const int f = j == beg ? kNegInf : x - (j - 1) * e_ins;
+        // columns without an in-band predecessor get -inf (this also keeps the
+        // subtraction away from INT32_MIN on out-of-band columns)
+        const int f = (j == beg || x == INT32_MIN) ? kNegInf : x - (j - 1) * e_ins;
         const int mm = M[c], e = E[c];
         int d = mm >= e ? 0 : 1;
         int hh = mm >= e ? mm : e;
@@ -160,46 +163,66 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
       if (A.score) A.score[task] = hq;
     }
     if (want_cigar) {
-      // The slot is written by every lane of this group and read back by its
-      // lane 0 below; all of them are lanes of one wave, so ordering the
-      // stores before the loads within the wave is all that is needed.
+      // The slot was written by every lane of this group; the walk below reads
+      // it back.  All lanes of the group run the walk (the loop is uniform over
+      // the group; lanes outside this task's group are not involved), so no
+      // lane-divergent loop is left in front of the next fetch broadcast; only
+      // lane 0 stores.  Runs of equal ops are merged in registers, which gives
+      // the CIGAR push_cigar builds (ksw.c:492-502).
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      if (lig == 0) {
-        uint32_t* cg = A.cigar + (int64_t)task * A.cigar_slot;
-        const int64_t cap = A.cigar_slot;
-        int n = 0, state = 0;
-        bool over = false;
-        int i = tlen - 1;
-        int k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
-        auto push = [&](int op, int len) {
-          if (n > 0 && (int)(cg[n - 1] & 0xf) == op) {
-            cg[n - 1] += (uint32_t)len << 4;
-          } else if (n < cap) {
-            cg[n++] = (uint32_t)len << 4 | (uint32_t)op;
-          } else {
-            over = true;
-          }
-        };
-        while (i >= 0 && k >= 0 && !over) {
-          const int lo = i > w ? i - w : 0;
-          if (k < lo || k - lo >= ncol) {  // cannot happen under the preconditions; never read outside the slot
-            over = true;
-            break;
-          }
-          state = z[(int64_t)i * ncol + (k - lo)] >> (state << 1) & 3;
-          if (state == 0) { push(0, 1); --i; --k; }
-          else if (state == 1) { push(2, 1); --i; }
-          else { push(1, 1); --k; }
+      uint32_t* cg = A.cigar + (int64_t)task * A.cigar_slot;
+      const int64_t cap = A.cigar_slot;
+      int n = 0, state = 0, run_op = -1, run_len = 0;
+      bool over = false;
+      auto emit = [&]() {
+        if (run_op < 0) return;
+        if (n < cap) {
+          if (lig == 0) cg[n] = (uint32_t)run_len << 4 | (uint32_t)run_op;
+          ++n;
+        } else {
+          over = true;
         }
-        if (!over && i >= 0) push(2, i + 1);
-        if (!over && k >= 0) push(1, k + 1);
-        if (over) {
+      };
+      auto push = [&](int op, int len) {
+        if (op == run_op) {
+          run_len += len;
+        } else {
+          emit();
+          run_op = op;
+          run_len = len;
+        }
+      };
+      int i = tlen - 1;
+      int k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
+      while (i >= 0 && k >= 0) {
+        const int lo = i > w ? i - w : 0;
+        if (k < lo || k - lo >= ncol) {  // cannot happen under the preconditions; never read outside the slot
+          over = true;
+          break;
+        }
+        // lane 0's byte, broadcast: every lane takes the same step by construction
+        const int zb = __shfl((int)z[(int64_t)i * ncol + (k - lo)], 0, L);
+        state = zb >> (state << 1) & 3;
+        if (state == 0) { push(0, 1); --i; --k; }
+        else if (state == 1) { push(2, 1); --i; }
+        else { push(1, 1); --k; }
+      }
+      if (i >= 0) push(2, i + 1);
+      if (k >= 0) push(1, k + 1);
+      emit();
+      if (over) {
+        if (lig == 0) {
           A.n_cigar[task] = -1;
           atomicAdd(&W.err[2], 1);
-        } else {
-          for (int a = 0, b = n - 1; a < b; ++a, --b) { const uint32_t tmp = cg[a]; cg[a] = cg[b]; cg[b] = tmp; }
-          A.n_cigar[task] = n;
         }
+      } else {
+        // reverse in place (the walk produced the ops from the end); uniform loop
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        for (int a = 0, b = n - 1; a < b; ++a, --b) {
+          const uint32_t ta = cg[a], tb = cg[b];
+          if (lig == 0) { cg[a] = tb; cg[b] = ta; }
+        }
+        if (lig == 0) A.n_cigar[task] = n;
       }
     } else if (A.n_cigar && lig == 0) {
       A.n_cigar[task] = 0;
