@@ -15,6 +15,9 @@
 
 using namespace bsx;
 
+// counts[kExtBins], cursor[kExtBins], err[4], fetch[kExtBins * kExtSlices]
+static constexpr size_t kSmallBytes = 4 * (2 * kExtBins + 4 + kExtBins * kExtSlices);
+
 namespace {
 
 // Grow-only device buffer.
@@ -47,7 +50,7 @@ struct bsx_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   // extension workspace
-  DevBuf bin_of, list, small;  // small: counts/cursor/fetch/err
+  DevBuf bin_of, list, desc, small;  // small: counts/cursor/err/fetch
   DevBuf zbuf;                 // direction-byte slots of the global kernel
   int64_t zslot = 1 << 16;     // bytes per slot: largest min(qlen,2w+1)*tlen accepted
   // staging for the host-memory entry points
@@ -145,7 +148,7 @@ void bsx_destroy(bsx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->bin_of, &c->list, &c->small, &c->zbuf, &c->h_query, &c->h_target, &c->h_qoff, &c->h_toff,
+  for (DevBuf* b : {&c->bin_of, &c->list, &c->desc, &c->small, &c->zbuf, &c->h_query, &c->h_target, &c->h_qoff, &c->h_toff,
                     &c->h_i32, &c->h_res})
     b->release();
   if (c->err_host) (void)hipHostFree(c->err_host);
@@ -201,15 +204,15 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   a.max_off = r->max_off; a.cells = r->cells;
   const size_t n = (size_t)t->n;
   HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
-  HIPCHK(c, c->list.ensure(4 * n + 4), "hipMalloc(list)");
-  HIPCHK(c, c->small.ensure(4 * (3 * kExtBins + 4)), "hipMalloc(work)");
+  HIPCHK(c, c->desc.ensure(sizeof(ExtDesc) * (n + 1)), "hipMalloc(desc)");
+  HIPCHK(c, c->small.ensure(kSmallBytes), "hipMalloc(work)");
   ExtWork wk;
   wk.bin_of = (uint8_t*)c->bin_of.p;
-  wk.list = (int32_t*)c->list.p;
+  wk.desc = (ExtDesc*)c->desc.p;
   wk.counts = (uint32_t*)c->small.p;
   wk.cursor = wk.counts + kExtBins;
-  wk.fetch = wk.cursor + kExtBins;
-  wk.err = (int32_t*)(wk.fetch + kExtBins);
+  wk.err = (int32_t*)(wk.cursor + kExtBins);
+  wk.fetch = (uint32_t*)(wk.err + 4);
   HIPCHK(c, launch_extend(a, wk, c->num_cu, c->stream, nullptr), "extension launch");
   HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
          "hipMemcpyAsync(err)");
@@ -373,14 +376,14 @@ int bsx_global_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tas
   const size_t n = (size_t)t->n;
   HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
   HIPCHK(c, c->list.ensure(4 * n + 4), "hipMalloc(list)");
-  HIPCHK(c, c->small.ensure(4 * (3 * kExtBins + 4)), "hipMalloc(work)");
+  HIPCHK(c, c->small.ensure(kSmallBytes), "hipMalloc(work)");
   GlbWork wk;
   wk.bin_of = (uint8_t*)c->bin_of.p;
   wk.list = (int32_t*)c->list.p;
   wk.counts = (uint32_t*)c->small.p;
   wk.cursor = wk.counts + kExtBins;
-  wk.fetch = wk.cursor + kExtBins;
-  wk.err = (int32_t*)(wk.fetch + kExtBins);
+  wk.err = (int32_t*)(wk.cursor + kExtBins);
+  wk.fetch = (uint32_t*)(wk.err + 4);
   wk.zslot = c->zslot;
   wk.z = nullptr;
   if (r->cigar) {

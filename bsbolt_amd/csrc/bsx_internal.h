@@ -11,7 +11,9 @@ namespace bsx {
 constexpr int kExtBins = 8;
 struct BinShape { int lanes, cols; };
 constexpr BinShape kExtShape[kExtBins] = {
-    {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
+    {16, 1}, {16, 2}, {16, 4}, {16, 8}, {16, 16}, {64, 8}, {64, 16}, {64, 32}};
+constexpr int kExtSlices = 8;  // work slices per bin (one per XCD)
+constexpr int kExtChunk = 4;   // task descriptors taken per fetch atomic
 constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
 
 // Scores are tracked as (h << 11 | column) keys in 32-bit lanes.
@@ -38,13 +40,21 @@ struct ExtArgs {
   int64_t* cells;
 };
 
+// One task of a bin's work list (written by the binning pass, in bin order).
+struct ExtDesc {
+  int32_t task, qlen, tlen, h0;
+  int32_t w, pad0;  // w already clamped as ksw.c:399-407
+  uint32_t qoff_lo, qoff_hi, toff_lo, toff_hi, pad1, pad2;
+};
+static_assert(sizeof(ExtDesc) == 48, "descriptor layout");
+
 // Per-call device workspace for the extension path.
 struct ExtWork {
   uint8_t* bin_of;     // [n] bin id, 0xff = rejected task
-  int32_t* list;       // [n] task indices grouped by bin
+  ExtDesc* desc;       // [n] task descriptors grouped by bin
   uint32_t* counts;    // [kExtBins] tasks per bin
   uint32_t* cursor;    // [kExtBins] scatter cursors
-  uint32_t* fetch;     // [kExtBins] persistent-kernel fetch counters
+  uint32_t* fetch;     // [kExtBins * kExtSlices] persistent-kernel fetch counters
   int32_t* err;        // [0] rejected count, [1] lowest rejected index
 };
 

@@ -1,115 +1,177 @@
 // Bit-exact batched ksw_extend2 (bsbolt/External/BWA/ksw.c:380-479) for gfx950.
 //
-// Layout: a task is owned by a group of L lanes of one wave64 (64/L tasks per
-// wave); lane g holds query columns [g*C, g*C+C) of the DP state in VGPRs:
+// Layout.  A task is owned by a group of L lanes (L = 16, 32 or 64; 64/L tasks
+// per wave64); lane g holds query columns [g*C, g*C+C) of the DP state in VGPRs:
 //   G[c] = eh[j].h = H(i-1, j-1)   (diagonal predecessor)
 //   E[c] = eh[j].e = E(i, j)
-// Rows (target bases) are processed one at a time, exactly like the reference
+// Rows (target bases) are processed one at a time exactly like the reference's
 // row loop, so the adaptive band [beg,end), the z-drop and the early exits are
-// decided on the same values.  Columns outside the active band are never
-// written, which reproduces the reference's reads of stale eh[] entries when
-// the band regrows (ksw.c:466-469).
+// decided on the same values.  Columns >= end are never written, which keeps
+// the stale eh[] entries the reference reads when the band regrows
+// (ksw.c:466-469).
 //
-// Within a row the only sequential dependency is F.  Because F is built from
-// M (not H), F(j) = max_{beg<=k<j}(max(M_k - oe_ins, 0) + k*e_ins) - (j-1)*e_ins
-// for j > beg and 0 at j = beg: an exclusive prefix-max over the row, done as a
-// lane-local scan plus a group scan.  Row maximum (ties -> larger j), the new
-// beg/end and the value at column end-1 are group reductions.
+// Row recurrences.  F is built from M, not H (ksw.c:433-447), so
+//   F(j) = max_{beg<=k<j}(max(M_k - oe_ins, 0) + k*e_ins) - (j-1)*e_ins  (j > beg)
+// and F(beg) = 0: an exclusive prefix maximum, computed as a lane-local scan
+// plus a DPP scan across the group.  Keys are shifted by +e_ins so that every
+// key is >= 1 and 0 is the scan identity (fused v_max_i32_dpp, bound_ctrl).
 //
-// The kernel is persistent: groups pull task indices from a per-bin counter and
-// run a flattened loop (fetch-or-row per iteration) so that groups of a wave do
-// not wait for one another between tasks.
+// Dead columns.  beg never decreases, so columns < beg are never read again.
+// The kernel keeps them at G = E = 0 (zeroing them when the band cap i-w moves
+// beg past non-zero columns).  A dead column then has M = 0, key = (k+1)*e_ins
+// and H = 0, which can neither raise F of a live column above its true value
+// (its key contributes (k-j+1)*e_ins <= 0 <= F(j)) nor change the row maximum,
+// the G shift or the zero scan.  So in-band masks are needed only against
+// columns >= end, and G(j) = H(i, j-1) holds for every j <= end with
+// H(i, -1) = the first-column value of ksw.c:421-424.
+//
+// Work distribution.  Tasks are binned by query length (bin b: qlen <= L*C);
+// the binning pass writes one 48-byte descriptor per task in bin order.  Each
+// bin's list is cut into 8 slices, one per XCD; groups take chunks of kChunk
+// descriptors with one atomic, from their block's slice first and then from
+// the others.  The next task's descriptor, query bases and first target word
+// are loaded one task ahead, so a task starts without a memory round trip.
+// The kernel is persistent: groups run a flattened loop (start-task-or-row per
+// iteration) so the groups of a wave never wait for one another between tasks.
 #include "bsx_internal.h"
 
 namespace bsx {
 
+// ---- DPP group primitives (identity 0, values >= 0) ----
 template <int L>
-__device__ __forceinline__ int group_max(int x) {
-#pragma unroll
-  for (int d = L / 2; d >= 1; d >>= 1) x = max(x, __shfl_xor(x, d, L));
+__device__ __forceinline__ int grp_incl_max(int x) {
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true));  // row_shr:1
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true));  // row_shr:2
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true));  // row_shr:4
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true));  // row_shr:8
+  if (L >= 32) x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  if (L >= 64) x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
   return x;
 }
 
+// exclusive version of an inclusive scan result (lane 0 of the group gets 0)
 template <int L>
-__device__ __forceinline__ int group_min(int x) {
-#pragma unroll
-  for (int d = L / 2; d >= 1; d >>= 1) x = min(x, __shfl_xor(x, d, L));
-  return x;
+__device__ __forceinline__ int grp_excl_from_incl(int incl, int lig) {
+  if (L == 16) return __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
+  const int e = __builtin_amdgcn_update_dpp(0, incl, 0x138, 0xf, 0xf, true);        // wave_shr:1
+  return lig == 0 ? 0 : e;
 }
 
-// Exclusive prefix max over the lanes of a group; lane 0 receives `ident`.
+// every lane of the group receives the maximum
 template <int L>
-__device__ __forceinline__ int group_excl_max(int x, int lig, int ident) {
-#pragma unroll
-  for (int d = 1; d < L; d <<= 1) {
-    int y = __shfl_up(x, d, L);
-    if (lig >= d) x = max(x, y);
+__device__ __forceinline__ int grp_max(int x, int lane) {
+  if (L == 16) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true));   // quad_perm [1,0,3,2]
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true));   // quad_perm [2,3,0,1]
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true));  // row_half_mirror
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true));  // row_mirror
+    return x;
   }
-  int e = __shfl_up(x, 1, L);
-  return lig == 0 ? ident : e;
+  x = grp_incl_max<L>(x);
+  if (L == 64) return __builtin_amdgcn_readlane(x, 63);
+  const int a = __builtin_amdgcn_readlane(x, 31), b = __builtin_amdgcn_readlane(x, 63);
+  return lane < 32 ? a : b;
 }
 
-__device__ __forceinline__ int prof_score(uint32_t lo, int hi, int t) {
-  // signed byte t of lo for t < 4, hi for t == 4
-  return t < 4 ? (int)((int32_t)(lo << (24 - 8 * t)) >> 24) : hi;
+__device__ __forceinline__ int sbyte(uint32_t w, int t) {  // signed byte t of w
+  return (int)((int32_t)(w << (24 - 8 * t)) >> 24);
 }
 
 template <int L, int C>
 __global__ void __launch_bounds__(256)
-ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t* __restrict__ counts,
+ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
                  int bin, uint32_t* __restrict__ fetch) {
   const int lane = threadIdx.x & 63;
   const int lig = lane & (L - 1);
   const int base = lig * C;
   uint32_t lbase = 0;
   for (int b = 0; b < bin; ++b) lbase += counts[b];
-  const int32_t* __restrict__ list = list_all + lbase;
   const uint32_t nlist = counts[bin];
+  if (nlist == 0) return;
+  const ExtDesc* __restrict__ desc = desc_all + lbase;
+  uint32_t* __restrict__ qctr = fetch + bin * kExtSlices;
   const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
   const int e_del = A.e_del, e_ins = A.e_ins;
 
-  int G[C], E[C];
+  // slice s covers [s*nlist/8, (s+1)*nlist/8)
+  int slice = blockIdx.x % kExtSlices, nsliced = 0;
+  uint32_t cpos = 0, cend = 0;  // current chunk [cpos, cend)
+
+  // next descriptor position for this group; nlist once everything is taken
+  auto next_pos = [&]() -> uint32_t {
+    while (cpos >= cend) {
+      if (nsliced >= kExtSlices) return nlist;
+      const uint32_t s0 = (uint32_t)(((uint64_t)slice * nlist) / kExtSlices);
+      const uint32_t s1 = (uint32_t)(((uint64_t)(slice + 1) * nlist) / kExtSlices);
+      uint32_t mine = 0;
+      if (lig == 0) mine = atomicAdd(qctr + slice, (uint32_t)kExtChunk);
+      const uint32_t got = __shfl(mine, 0, L);
+      if (s0 + got < s1) {
+        cpos = s0 + got;
+        cend = min(cpos + (uint32_t)kExtChunk, s1);
+      } else {
+        slice = (slice + 1) % kExtSlices;
+        ++nsliced;
+      }
+    }
+    return cpos++;
+  };
+
+  // ---- prefetched next task ----
+  bool nx_ok = false;
+  int nx_task = 0, nx_qlen = 0, nx_tlen = 0, nx_h0 = 0, nx_w = 0;
+  uint64_t nx_toff = 0;
+  uint32_t nx_q[C];    // query code of each owned column (4 = padding)
+  uint32_t nx_tw = 0;  // target bases [4*lig, 4*lig+4) of the next task
+  auto prefetch = [&]() {
+    const uint32_t p = next_pos();
+    nx_ok = p < nlist;
+    if (!nx_ok) return;
+    const int4* d = (const int4*)(desc + p);
+    const int4 d0 = d[0], d1 = d[1], d2 = d[2];
+    nx_task = d0.x; nx_qlen = d0.y; nx_tlen = d0.z; nx_h0 = d0.w;
+    nx_w = d1.x;
+    const uint64_t qoff = (uint64_t)(uint32_t)d1.z | ((uint64_t)(uint32_t)d1.w << 32);
+    nx_toff = (uint64_t)(uint32_t)d2.x | ((uint64_t)(uint32_t)d2.y << 32);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int j = base + c;
+      nx_q[c] = j < nx_qlen ? (uint32_t)A.query[qoff + j] : 4u;
+    }
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int r = 4 * lig + k;
+      v |= (r < nx_tlen ? (uint32_t)A.target[nx_toff + r] : 0u) << (8 * k);
+    }
+    nx_tw = v;
+  };
+
+  // ---- current task ----
+  int G[C], E[C], kE[C];
   uint32_t plo[C];
   int phi[C];
 #pragma unroll
-  for (int c = 0; c < C; ++c) { G[c] = 0; E[c] = 0; plo[c] = 0; phi[c] = 0; }
-
+  for (int c = 0; c < C; ++c) { G[c] = 0; E[c] = 0; kE[c] = 0; plo[c] = 0; phi[c] = 0; nx_q[c] = 4; }
   int task = 0, qlen = 0, tlen = 0, h0 = 0, w = 0, i = 0, beg = 0, end = 0;
   int best = 0, best_i = -1, best_j = -1, gbest = -1, gbest_i = -1, moff = 0;
-  uint32_t ncell = 0;
+  uint32_t ncell = 0, tch = 0;
   uint64_t toff = 0;
-  uint32_t tch = 0;
   int tcb = 0;
   bool need = true, done = false;
 
+  prefetch();
   for (;;) {
     if (need && !done) {
-      // Branch-free fetch: every lane of the group issues the atomic (only lane
-      // 0 adds 1), so no lane-divergent region precedes the broadcast.
-      const uint32_t mine = atomicAdd(fetch, lig == 0 ? 1u : 0u);
-      const uint32_t idx = __shfl(mine, 0, L);
-      if (idx >= nlist) {
+      if (!nx_ok) {
         done = true;
       } else {
-        task = list[idx];
-        qlen = A.qlen[task];
-        tlen = A.tlen[task];
-        h0 = A.h0[task];
-        w = A.w[task];
-        const int eb = A.end_bonus[task];
-        const uint64_t qoff = A.q_off[task];
-        toff = A.t_off[task];
-        // band clamp of ksw.c:399-407, in the same double arithmetic
-        int lim = (int)((double)(qlen * A.mat_max + eb - A.o_ins) / e_ins + 1.);
-        lim = lim > 1 ? lim : 1;
-        w = w < lim ? w : lim;
-        lim = (int)((double)(qlen * A.mat_max + eb - A.o_del) / e_del + 1.);
-        lim = lim > 1 ? lim : 1;
-        w = w < lim ? w : lim;
+        task = nx_task; qlen = nx_qlen; tlen = nx_tlen; h0 = nx_h0; w = nx_w; toff = nx_toff;
+        tch = nx_tw; tcb = 0;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const int j = base + c;
-          const int qc = j < qlen ? (int)A.query[qoff + j] : 4;
+          const uint32_t qc = nx_q[c];
           uint32_t lo = A.prof_lo[4];
           int hi = A.prof_hi[4];
           if (qc == 0) { lo = A.prof_lo[0]; hi = A.prof_hi[0]; }
@@ -118,15 +180,16 @@ ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t
           if (qc == 3) { lo = A.prof_lo[3]; hi = A.prof_hi[3]; }
           plo[c] = lo;
           phi[c] = hi;
+          kE[c] = j * e_ins;
           // first row (ksw.c:395-397): eh[0].h = h0, eh[j].h = max(h0-oe_ins-(j-1)e_ins, 0)
-          int g = j == 0 ? h0 : h0 - oe_ins - (j - 1) * e_ins;
+          const int g = j == 0 ? h0 : h0 - oe_ins - (j - 1) * e_ins;
           G[c] = g > 0 ? g : 0;
           E[c] = 0;
         }
         i = 0; beg = 0; end = qlen;
         best = h0; best_i = best_j = -1; gbest = -1; gbest_i = -1; moff = 0;
         ncell = 0;
-        tcb = -(1 << 30);
+        prefetch();
         if (tlen <= 0) {
           if (lig == 0) {
             if (A.score) A.score[task] = h0;
@@ -146,21 +209,27 @@ ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t
     if (need || done) continue;
 
     // ---------------- one DP row (ksw.c:412-470) ----------------
-    if (i - tcb >= 4 * L) {  // refill the group's window of 4*L target bases
+    if (i - tcb >= 4 * L) {  // next window of 4*L target bases
       tcb = i;
       uint32_t v = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int r = i + 4 * lig + k;
-        const uint32_t b = r < tlen ? (uint32_t)A.target[toff + r] : 0u;
-        v |= b << (8 * k);
+        v |= (r < tlen ? (uint32_t)A.target[toff + r] : 0u) << (8 * k);
       }
       tch = v;
     }
     const int tr = i - tcb;
     const int t = (int)((__shfl(tch, tr >> 2, L) >> (8 * (tr & 3))) & 0xffu);
+    const bool tN = t > 3;
 
-    if (beg < i - w) beg = i - w;
+    const int cap_beg = i - w;
+    if (beg < cap_beg) {  // the band cap moves beg: keep the dead-column invariant
+      beg = cap_beg;
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        if (base + c < beg) { G[c] = 0; E[c] = 0; }
+    }
     if (end > i + w + 1) end = i + w + 1;
     if (end > qlen) end = qlen;
     int h1p = 0;
@@ -170,63 +239,55 @@ ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t
     }
 
     int M[C], ex[C];
-    int run = -1;
+    int run = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int j = base + c;
-      const bool inb = (j >= beg) & (j < end);
-      const int s = prof_score(plo[c], phi[c], t);
+      const int s = tN ? phi[c] : sbyte(plo[c], t & 3);
       const int g = G[c];
       const int mm = g ? g + s : 0;
       M[c] = mm;
-      int tt = mm - oe_ins;
-      tt = tt > 0 ? tt : 0;
-      const int key = inb ? tt + j * e_ins : -1;
+      // key + e_ins = max(M - oe_ins, 0) + (j+1)*e_ins  (>= 1)
+      const int key = max(mm - oe_ins, 0) + kE[c] + e_ins;
       ex[c] = run;
       run = max(run, key);
     }
-    const int lex = group_excl_max<L>(run, lig, -1);
+    const int lex = grp_excl_from_incl<L>(grp_incl_max<L>(run), lig);
 
-    int h[C], nE[C];
-    int mk = -1, h1c = -1;
+    int h[C];
+    int mk = 0, h1c = 0, hik = 0, lok = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int j = base + c;
-      const bool inb = (j >= beg) & (j < end);
+      const bool live = j < end;
       const int x = max(ex[c], lex);
-      const int f = x < 0 ? 0 : x - (j - 1) * e_ins;
-      int hh = max(M[c], E[c]);
-      hh = max(hh, f);
+      const int f = x - kE[c];  // F(j); 0 when no column precedes j
+      const int hh = max(max(M[c], E[c]), f);
       h[c] = hh;
-      int od = M[c] - oe_del;
-      od = od > 0 ? od : 0;
-      const int ne = max(E[c] - e_del, od);
-      nE[c] = ne;
-      if (inb) {
+      const int ne = max(max(E[c] - e_del, M[c] - oe_del), 0);
+      if (live) {
         mk = max(mk, (hh << 11) | j);
-        if (j == end - 1) h1c = hh;
+        E[c] = ne;
       }
+      if (j == end - 1) h1c = hh;
     }
-    const int hprev = __shfl_up(h[C - 1], 1, L);
-    int lo = 0x7fffffff, hi = -1;
+    // G(j) = H(i, j-1) for j <= end; column 0 receives the first-column value
+    const int hsh = __builtin_amdgcn_update_dpp(0, h[C - 1], L == 16 ? 0x111 : 0x138, 0xf, 0xf, true);
+    const int hprev = lig == 0 ? h1p : hsh;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int j = base + c;
-      const bool inb = (j >= beg) & (j < end);
       const int left = c == 0 ? hprev : h[c - 1];
-      const int ng = j == beg ? h1p : left;
-      if ((j >= beg) & (j <= end) & (j < qlen)) G[c] = ng;
-      if (inb) E[c] = nE[c];
-      else if (j == end) E[c] = 0;
-      if (inb & ((G[c] | E[c]) != 0)) {
-        lo = min(lo, j);
-        hi = max(hi, j);
+      if (j <= end) G[c] = left;
+      if (j == end) E[c] = 0;
+      if ((j < end) & ((G[c] | E[c]) != 0)) {
+        hik = max(hik, j + 1);
+        lok = max(lok, 2048 - j);
       }
     }
-    mk = group_max<L>(mk);
-    h1c = group_max<L>(h1c);
-    lo = group_min<L>(lo);
-    hi = group_max<L>(hi);
+    mk = grp_max<L>(mk, lane);
+    h1c = grp_max<L>(h1c, lane);
+    hik = grp_max<L>(hik, lane);
+    lok = grp_max<L>(lok, lane);
 
     const int h1 = end > beg ? h1c : h1p;
     const int jafter = end > beg ? end : beg;
@@ -235,8 +296,8 @@ ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t
       gbest = gbest > h1 ? gbest : h1;
     }
     ncell += end > beg ? (uint32_t)(end - beg) : 0u;
-    const int m = mk >= 0 ? (mk >> 11) : 0;
-    const int mj = mk >= 0 ? (mk & 2047) : -1;
+    const int m = end > beg ? mk >> 11 : 0;
+    const int mj = mk & 2047;
     bool fin = false;
     if (m == 0) {
       fin = true;
@@ -250,8 +311,8 @@ ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t
       if (drop > A.zdrop) fin = true;
     }
     if (!fin) {  // ksw.c:466-469
-      const int nb = lo == 0x7fffffff ? end : lo;
-      const int hz = h1 != 0 ? end : hi;
+      const int nb = lok > 0 ? 2048 - lok : end;
+      const int hz = h1 != 0 ? end : hik - 1;
       beg = nb;
       const int ne = hz >= 0 ? hz + 2 : nb + 1;
       end = ne < qlen ? ne : qlen;
@@ -273,7 +334,7 @@ ext_plain_kernel(ExtArgs A, const int32_t* __restrict__ list_all, const uint32_t
   }
 }
 
-// ---- binning: validate every task and group task indices by query length ----
+// ---- binning: validate every task, group tasks by query length, write descriptors ----
 
 __device__ __forceinline__ int ext_bin_of(int qlen) {
 #pragma unroll
@@ -287,6 +348,7 @@ ext_bin_count_kernel(ExtArgs A, ExtWork W) {
   __shared__ uint32_t hist[kExtBins];
   if (threadIdx.x < kExtBins) hist[threadIdx.x] = 0;
   __syncthreads();
+  int bad = 0, first_bad = 0x7fffffff;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int ql = A.qlen[t], tl = A.tlen[t], h0 = A.h0[t], w = A.w[t], eb = A.end_bonus[t];
@@ -295,37 +357,68 @@ ext_bin_count_kernel(ExtArgs A, ExtWork W) {
                     eb < kScoreLimit && (int64_t)h0 + (int64_t)ql * A.mat_max < kScoreLimit;
     if (!ok) {
       b = 0xff;
-      atomicAdd(&W.err[0], 1);
-      atomicMin(&W.err[1], (int)t);
+      ++bad;
+      first_bad = min(first_bad, (int)t);
     } else {
       atomicAdd(&hist[b], 1u);
     }
     W.bin_of[t] = (uint8_t)b;
   }
+  if (bad) {
+    atomicAdd(&W.err[0], bad);
+    atomicMin(&W.err[1], first_bad);
+  }
   __syncthreads();
   if (threadIdx.x < kExtBins && hist[threadIdx.x]) atomicAdd(&W.counts[threadIdx.x], hist[threadIdx.x]);
 }
 
+// Each block handles a contiguous range of tasks: it counts them per bin in
+// LDS, reserves one range per bin with a single global atomic, and writes the
+// descriptors of its tasks into those ranges.
 __global__ void __launch_bounds__(256)
-ext_bin_scatter_kernel(int64_t n, ExtWork W) {
-  __shared__ uint32_t off[kExtBins];
-  if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (int b = 0; b < kExtBins; ++b) { off[b] = s; s += W.counts[b]; }
+ext_bin_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
+  __shared__ uint32_t off[kExtBins], hist[kExtBins], cur[kExtBins];
+  const int64_t t0 = (int64_t)blockIdx.x * per_block;
+  const int64_t t1 = min(A.n, t0 + per_block);
+  if (threadIdx.x < kExtBins) { hist[threadIdx.x] = 0; cur[threadIdx.x] = 0; }
+  __syncthreads();
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const int b = W.bin_of[t];
+    if (b != 0xff) atomicAdd(&hist[b], 1u);
   }
   __syncthreads();
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  if (threadIdx.x < kExtBins) {
+    uint32_t s = 0;
+    for (int b = 0; b < (int)threadIdx.x; ++b) s += W.counts[b];
+    const uint32_t hcount = hist[threadIdx.x];
+    off[threadIdx.x] = s + (hcount ? atomicAdd(&W.cursor[threadIdx.x], hcount) : 0u);
+  }
+  __syncthreads();
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int b = W.bin_of[t];
     if (b == 0xff) continue;
-    const uint32_t p = atomicAdd(&W.cursor[b], 1u);
-    W.list[off[b] + p] = (int32_t)t;
+    const uint32_t p = off[b] + atomicAdd(&cur[b], 1u);
+    const int ql = A.qlen[t], eb = A.end_bonus[t];
+    int w = A.w[t];
+    // band clamp of ksw.c:399-407, in the same double arithmetic
+    int lim = (int)((double)(ql * A.mat_max + eb - A.o_ins) / A.e_ins + 1.);
+    lim = lim > 1 ? lim : 1;
+    w = w < lim ? w : lim;
+    lim = (int)((double)(ql * A.mat_max + eb - A.o_del) / A.e_del + 1.);
+    lim = lim > 1 ? lim : 1;
+    w = w < lim ? w : lim;
+    const uint64_t qo = A.q_off[t], to = A.t_off[t];
+    int4* d = (int4*)(W.desc + p);
+    d[0] = make_int4((int)t, ql, A.tlen[t], A.h0[t]);
+    d[1] = make_int4(w, 0, (int)(uint32_t)qo, (int)(uint32_t)(qo >> 32));
+    d[2] = make_int4((int)(uint32_t)to, (int)(uint32_t)(to >> 32), 0, 0);
   }
 }
 
 __global__ void ext_reset_kernel(ExtWork W) {
   const int k = threadIdx.x;
-  if (k < kExtBins) { W.counts[k] = 0; W.cursor[k] = 0; W.fetch[k] = 0; }
+  if (k < kExtBins) { W.counts[k] = 0; W.cursor[k] = 0; }
+  if (k < kExtBins * kExtSlices) W.fetch[k] = 0;
   if (k == 0) { W.err[0] = 0; W.err[1] = 0x7fffffff; }
 }
 
@@ -341,17 +434,18 @@ static void launch_bin(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStrea
   }
   if (blocks < 1) return;
   hipLaunchKernelGGL((ext_plain_kernel<L, C>), dim3((unsigned)blocks), dim3(256), 0, st, a,
-                     (const int32_t*)wk.list, (const uint32_t*)wk.counts, B, wk.fetch + B);
+                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B, wk.fetch);
 }
 
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts) {
-  hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(64), 0, st, wk);
+  hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(kExtBins * kExtSlices), 0, st, wk);
   if (a.n <= 0) return hipGetLastError();
   int64_t blocks = (a.n + 255) / 256;
   if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
   hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk);
-  hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a.n, wk);
+  const int64_t per_block = (a.n + blocks - 1) / blocks;
+  hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
   // longest queries first so the long tasks do not form the tail
 #define BSX_BIN(B) launch_bin<B>(a, wk, num_cu, st, host_bin_counts ? host_bin_counts[B] : -1)
   BSX_BIN(7); BSX_BIN(6); BSX_BIN(5); BSX_BIN(4); BSX_BIN(3); BSX_BIN(2); BSX_BIN(1); BSX_BIN(0);
