@@ -12,8 +12,8 @@ constexpr int kExtBins = 8;
 struct BinShape { int lanes, cols; };
 constexpr BinShape kExtShape[kExtBins] = {
     {16, 1}, {16, 2}, {16, 4}, {16, 8}, {16, 16}, {64, 8}, {64, 16}, {64, 32}};
-constexpr int kExtSlices = 8;  // work slices per bin (one per XCD)
-constexpr int kExtChunk = 4;   // task descriptors taken per fetch atomic
+constexpr int kExtSlices = 8;           // fetch-counter slots per bin (CIGAR path)
+constexpr int kExtTasksPerBlock = 128;  // task descriptors owned by one extension block
 constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
 
 // Scores are tracked as (h << 11 | column) keys in 32-bit lanes.

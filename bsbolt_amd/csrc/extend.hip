@@ -26,13 +26,15 @@
 // H(i, -1) = the first-column value of ksw.c:421-424.
 //
 // Work distribution.  Tasks are binned by query length (bin b: qlen <= L*C);
-// the binning pass writes one 48-byte descriptor per task in bin order.  Each
-// bin's list is cut into 8 slices, one per XCD; groups take chunks of kChunk
-// descriptors with one atomic, from their block's slice first and then from
-// the others.  The next task's descriptor, query bases and first target word
-// are loaded one task ahead, so a task starts without a memory round trip.
-// The kernel is persistent: groups run a flattened loop (start-task-or-row per
-// iteration) so the groups of a wave never wait for one another between tasks.
+// the binning pass writes one 48-byte descriptor per task in bin order.  Block
+// k of a bin's launch owns descriptors [k*T, k*T+T); its groups take them one
+// at a time through an LDS counter, so no device-scope atomic is on the task
+// path (same-address device atomics serialise at a few per microsecond) and the
+// hardware dispatcher balances the blocks.  The next task's descriptor, query
+// bases and first target word are loaded one task ahead, so a task starts
+// without a memory round trip.  Groups run a flattened loop (start-task-or-row
+// per iteration) so the groups of a wave never wait for one another between
+// tasks.
 #include "bsx_internal.h"
 
 namespace bsx {
@@ -80,41 +82,29 @@ __device__ __forceinline__ int sbyte(uint32_t w, int t) {  // signed byte t of w
 template <int L, int C>
 __global__ void __launch_bounds__(256)
 ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
-                 int bin, uint32_t* __restrict__ fetch) {
+                 int bin) {
+  __shared__ uint32_t lds_next;
   const int lane = threadIdx.x & 63;
   const int lig = lane & (L - 1);
   const int base = lig * C;
   uint32_t lbase = 0;
   for (int b = 0; b < bin; ++b) lbase += counts[b];
   const uint32_t nlist = counts[bin];
-  if (nlist == 0) return;
+  const uint32_t blk0 = blockIdx.x * (uint32_t)kExtTasksPerBlock;
+  if (blk0 >= nlist) return;  // uniform over the block
+  const uint32_t blk1 = min(blk0 + (uint32_t)kExtTasksPerBlock, nlist);
+  if (threadIdx.x == 0) lds_next = blk0;
+  __syncthreads();
   const ExtDesc* __restrict__ desc = desc_all + lbase;
-  uint32_t* __restrict__ qctr = fetch + bin * kExtSlices;
   const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
   const int e_del = A.e_del, e_ins = A.e_ins;
 
-  // slice s covers [s*nlist/8, (s+1)*nlist/8)
-  int slice = blockIdx.x % kExtSlices, nsliced = 0;
-  uint32_t cpos = 0, cend = 0;  // current chunk [cpos, cend)
-
-  // next descriptor position for this group; nlist once everything is taken
+  // next descriptor position for this group; blk1 once the block's range is taken
   auto next_pos = [&]() -> uint32_t {
-    while (cpos >= cend) {
-      if (nsliced >= kExtSlices) return nlist;
-      const uint32_t s0 = (uint32_t)(((uint64_t)slice * nlist) / kExtSlices);
-      const uint32_t s1 = (uint32_t)(((uint64_t)(slice + 1) * nlist) / kExtSlices);
-      uint32_t mine = 0;
-      if (lig == 0) mine = atomicAdd(qctr + slice, (uint32_t)kExtChunk);
-      const uint32_t got = __shfl(mine, 0, L);
-      if (s0 + got < s1) {
-        cpos = s0 + got;
-        cend = min(cpos + (uint32_t)kExtChunk, s1);
-      } else {
-        slice = (slice + 1) % kExtSlices;
-        ++nsliced;
-      }
-    }
-    return cpos++;
+    uint32_t mine = 0;
+    if (lig == 0) mine = atomicAdd(&lds_next, 1u);
+    const uint32_t p = __shfl(mine, 0, L);
+    return p < blk1 ? p : blk1;
   };
 
   // ---- prefetched next task ----
@@ -125,7 +115,7 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
   uint32_t nx_tw = 0;  // target bases [4*lig, 4*lig+4) of the next task
   auto prefetch = [&]() {
     const uint32_t p = next_pos();
-    nx_ok = p < nlist;
+    nx_ok = p < blk1;
     if (!nx_ok) return;
     const int4* d = (const int4*)(desc + p);
     const int4 d0 = d[0], d1 = d[1], d2 = d[2];
@@ -423,18 +413,15 @@ __global__ void ext_reset_kernel(ExtWork W) {
 }
 
 template <int B>
-static void launch_bin(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st, int64_t hint) {
+static void launch_bin(const ExtArgs& a, const ExtWork& wk, hipStream_t st, int64_t hint) {
   constexpr int L = kExtShape[B].lanes, C = kExtShape[B].cols;
-  constexpr int groups_per_block = 256 / L;
-  // persistent grid: enough blocks to fill the chip, fewer for small bins
-  int64_t blocks = (int64_t)num_cu * 8;
-  if (hint >= 0) {
-    const int64_t need = (hint + groups_per_block - 1) / groups_per_block;
-    if (need < blocks) blocks = need;
-  }
+  // one block per kExtTasksPerBlock descriptors of the bin; without a host-side
+  // count the grid covers the whole batch and surplus blocks exit at once
+  const int64_t n = hint >= 0 ? hint : a.n;
+  const int64_t blocks = (n + kExtTasksPerBlock - 1) / kExtTasksPerBlock;
   if (blocks < 1) return;
   hipLaunchKernelGGL((ext_plain_kernel<L, C>), dim3((unsigned)blocks), dim3(256), 0, st, a,
-                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B, wk.fetch);
+                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
 }
 
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
@@ -447,7 +434,7 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   const int64_t per_block = (a.n + blocks - 1) / blocks;
   hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
   // longest queries first so the long tasks do not form the tail
-#define BSX_BIN(B) launch_bin<B>(a, wk, num_cu, st, host_bin_counts ? host_bin_counts[B] : -1)
+#define BSX_BIN(B) launch_bin<B>(a, wk, st, host_bin_counts ? host_bin_counts[B] : -1)
   BSX_BIN(7); BSX_BIN(6); BSX_BIN(5); BSX_BIN(4); BSX_BIN(3); BSX_BIN(2); BSX_BIN(1); BSX_BIN(0);
 #undef BSX_BIN
   return hipGetLastError();
