@@ -50,7 +50,7 @@ struct bsx_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   // extension workspace
-  DevBuf bin_of, list, desc, small;  // small: counts/cursor/err/fetch
+  DevBuf bin_of, list, desc, rec, small;  // small: counts/cursor/err/fetch
   DevBuf zbuf;                 // direction-byte slots of the global kernel
   int64_t zslot = 1 << 16;     // bytes per slot: largest min(qlen,2w+1)*tlen accepted
   // staging for the host-memory entry points
@@ -148,7 +148,7 @@ void bsx_destroy(bsx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->bin_of, &c->list, &c->desc, &c->small, &c->zbuf, &c->h_query, &c->h_target, &c->h_qoff, &c->h_toff,
+  for (DevBuf* b : {&c->bin_of, &c->list, &c->desc, &c->rec, &c->small, &c->zbuf, &c->h_query, &c->h_target, &c->h_qoff, &c->h_toff,
                     &c->h_i32, &c->h_res})
     b->release();
   if (c->err_host) (void)hipHostFree(c->err_host);
@@ -205,10 +205,12 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   const size_t n = (size_t)t->n;
   HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
   HIPCHK(c, c->desc.ensure(sizeof(ExtDesc) * (n + 1)), "hipMalloc(desc)");
+  HIPCHK(c, c->rec.ensure(32 * (n + 1)), "hipMalloc(rec)");
   HIPCHK(c, c->small.ensure(kSmallBytes), "hipMalloc(work)");
   ExtWork wk;
   wk.bin_of = (uint8_t*)c->bin_of.p;
   wk.desc = (ExtDesc*)c->desc.p;
+  wk.rec = (int32_t*)c->rec.p;
   wk.counts = (uint32_t*)c->small.p;
   wk.cursor = wk.counts + kExtBins;
   wk.err = (int32_t*)(wk.cursor + kExtBins);

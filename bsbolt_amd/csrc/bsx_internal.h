@@ -11,7 +11,7 @@ namespace bsx {
 constexpr int kExtBins = 8;
 struct BinShape { int lanes, cols; };
 constexpr BinShape kExtShape[kExtBins] = {
-    {16, 1}, {16, 2}, {16, 4}, {16, 8}, {16, 16}, {64, 8}, {64, 16}, {64, 32}};
+    {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
 constexpr int kExtSlices = 8;           // fetch-counter slots per bin (CIGAR path)
 constexpr int kExtTasksPerBlock = 128;  // task descriptors owned by one extension block
 constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
@@ -52,6 +52,7 @@ static_assert(sizeof(ExtDesc) == 48, "descriptor layout");
 struct ExtWork {
   uint8_t* bin_of;     // [n] bin id, 0xff = rejected task
   ExtDesc* desc;       // [n] task descriptors grouped by bin
+  int32_t* rec;        // [n][8] packed per-task results
   uint32_t* counts;    // [kExtBins] tasks per bin
   uint32_t* cursor;    // [kExtBins] scatter cursors
   uint32_t* fetch;     // [kExtBins * kExtSlices] persistent-kernel fetch counters

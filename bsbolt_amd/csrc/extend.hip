@@ -79,9 +79,20 @@ __device__ __forceinline__ int sbyte(uint32_t w, int t) {  // signed byte t of w
   return (int)((int32_t)(w << (24 - 8 * t)) >> 24);
 }
 
-template <int L, int C>
+// Kernel-side arguments of the extension kernels: only what the row loop and
+// the task start read (the output arrays are written by ext_unpack_kernel).
+struct ExtKArgs {
+  const uint8_t* query;
+  const uint8_t* target;
+  int32_t* rec;  // [n][8]: score qle tle gtle gscore max_off cells_lo cells_hi
+  int o_del, e_del, o_ins, e_ins, zdrop;
+  uint32_t prof_lo[5];
+  int32_t prof_hi[5];
+};
+
+template <int L, int C, bool E1>
 __global__ void __launch_bounds__(256)
-ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
+ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
                  int bin) {
   __shared__ uint32_t lds_next;
   const int lane = threadIdx.x & 63;
@@ -96,8 +107,10 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
   if (threadIdx.x == 0) lds_next = blk0;
   __syncthreads();
   const ExtDesc* __restrict__ desc = desc_all + lbase;
-  const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
-  const int e_del = A.e_del, e_ins = A.e_ins;
+  // E1: both gap extensions are 1 (BSBolt's -E 1,1), so the per-column
+  // multiples of e_ins fold into immediates
+  const int e_del = E1 ? 1 : A.e_del, e_ins = E1 ? 1 : A.e_ins;
+  const int oe_del = A.o_del + e_del, oe_ins = A.o_ins + e_ins;
 
   // next descriptor position for this group; blk1 once the block's range is taken
   auto next_pos = [&]() -> uint32_t {
@@ -107,70 +120,96 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
     return p < blk1 ? p : blk1;
   };
 
-  // ---- prefetched next task ----
+  // ---- two-stage prefetch ----
+  // stage 2: descriptor of the task after next (loaded two tasks ahead)
+  // stage 1: descriptor fields + query/target bytes of the next task
+  // Every load is unconditional with a clamped in-bounds address, so nothing
+  // waits on it until the next task starts.
+  bool d2_ok = false;
+  int4 d2a = make_int4(0, 0, 0, 0), d2b = d2a, d2c = d2a;
+  auto load_desc = [&]() {
+    const uint32_t p = next_pos();
+    d2_ok = p < blk1;
+    const int4* d = (const int4*)(desc + (d2_ok ? p : blk0));
+    d2a = d[0]; d2b = d[1]; d2c = d[2];
+  };
   bool nx_ok = false;
   int nx_task = 0, nx_qlen = 0, nx_tlen = 0, nx_h0 = 0, nx_w = 0;
-  uint64_t nx_toff = 0;
-  uint32_t nx_q[C];    // query code of each owned column (4 = padding)
-  uint32_t nx_tw = 0;  // target bases [4*lig, 4*lig+4) of the next task
-  auto prefetch = [&]() {
-    const uint32_t p = next_pos();
-    nx_ok = p < blk1;
-    if (!nx_ok) return;
-    const int4* d = (const int4*)(desc + p);
-    const int4 d0 = d[0], d1 = d[1], d2 = d[2];
-    nx_task = d0.x; nx_qlen = d0.y; nx_tlen = d0.z; nx_h0 = d0.w;
-    nx_w = d1.x;
-    const uint64_t qoff = (uint64_t)(uint32_t)d1.z | ((uint64_t)(uint32_t)d1.w << 32);
-    nx_toff = (uint64_t)(uint32_t)d2.x | ((uint64_t)(uint32_t)d2.y << 32);
+  uint64_t nx_toff = 0, nx_qoff = 0;
+  uint32_t nx_q[C];   // raw query bytes of the owned columns (clamped index)
+  uint32_t nx_tb[4];  // raw target bytes [4*lig, 4*lig+4) (clamped index)
+  auto stage1 = [&]() {  // descriptor d2 -> next task, issue its data loads
+    nx_ok = d2_ok;
+    nx_task = d2a.x; nx_qlen = d2a.y; nx_tlen = d2a.z; nx_h0 = d2a.w;
+    nx_w = d2b.x;
+    const uint64_t qoff = (uint64_t)(uint32_t)d2b.z | ((uint64_t)(uint32_t)d2b.w << 32);
+    nx_toff = (uint64_t)(uint32_t)d2c.x | ((uint64_t)(uint32_t)d2c.y << 32);
+    nx_qoff = qoff;
+    const int qmax = nx_qlen > 0 ? nx_qlen - 1 : 0;
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const int j = base + c;
-      nx_q[c] = j < nx_qlen ? (uint32_t)A.query[qoff + j] : 4u;
-    }
-    uint32_t v = 0;
+    for (int c = 0; c < C; ++c) nx_q[c] = A.query[qoff + min(base + c, qmax)];
+    // a task without target bases reads its first query base instead (always valid)
+    const uint8_t* tp = nx_tlen > 0 ? A.target + nx_toff : A.query + qoff;
+    const int tmax = nx_tlen > 0 ? nx_tlen - 1 : 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int r = 4 * lig + k;
-      v |= (r < nx_tlen ? (uint32_t)A.target[nx_toff + r] : 0u) << (8 * k);
-    }
-    nx_tw = v;
+    for (int k2 = 0; k2 < 4; ++k2) nx_tb[k2] = tp[min(4 * lig + k2, tmax)];
+  };
+  // 4*L-base target window of the running task, double-buffered.  Always
+  // issued (clamped index; a task without target bases reads its first query
+  // base), so the prefetched registers never merge with a not-loaded path.
+  auto load_window = [&](const uint8_t* tp, int tlen_, int first, uint32_t* out) {
+    const int tmax = tlen_ > 0 ? tlen_ - 1 : 0;
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) out[k2] = tp[min(first + 4 * lig + k2, tmax)];
   };
 
   // ---- current task ----
-  int G[C], E[C], kE[C];
+  int G[C], E[C];
   uint32_t plo[C];
-  int phi[C];
+  uint32_t qpk[(C + 3) / 4];  // query codes, 4 per register (for rows whose target base is N)
 #pragma unroll
-  for (int c = 0; c < C; ++c) { G[c] = 0; E[c] = 0; kE[c] = 0; plo[c] = 0; phi[c] = 0; nx_q[c] = 4; }
+  for (int c = 0; c < C; ++c) { G[c] = 0; E[c] = 0; plo[c] = 0; nx_q[c] = 4; }
+#pragma unroll
+  for (int c = 0; c < (C + 3) / 4; ++c) qpk[c] = 0;
   int task = 0, qlen = 0, tlen = 0, h0 = 0, w = 0, i = 0, beg = 0, end = 0;
   int best = 0, best_i = -1, best_j = -1, gbest = -1, gbest_i = -1, moff = 0;
   uint32_t ncell = 0, tch = 0;
+  uint32_t tnx[4] = {0, 0, 0, 0};  // raw bytes of the following window
   uint64_t toff = 0;
+  const uint8_t* tbase = A.target;  // target bases of the running task (or a valid dummy)
   int tcb = 0;
   bool need = true, done = false;
+  const int kE0 = base * e_ins;  // column j contributes j*e_ins = kE0 + c*e_ins
 
-  prefetch();
+  load_desc();
+  stage1();
+  load_desc();
   for (;;) {
     if (need && !done) {
       if (!nx_ok) {
         done = true;
       } else {
         task = nx_task; qlen = nx_qlen; tlen = nx_tlen; h0 = nx_h0; w = nx_w; toff = nx_toff;
-        tch = nx_tw; tcb = 0;
+        {
+          uint32_t v = 0;
+#pragma unroll
+          for (int k2 = 0; k2 < 4; ++k2) v |= (4 * lig + k2 < tlen ? (nx_tb[k2] & 0xffu) : 0u) << (8 * k2);
+          tch = v;
+        }
+        tcb = 0;
+#pragma unroll
+        for (int c = 0; c < (C + 3) / 4; ++c) qpk[c] = 0;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const int j = base + c;
-          const uint32_t qc = nx_q[c];
+          const uint32_t qc = j < qlen ? (nx_q[c] & 0xffu) : 4u;
           uint32_t lo = A.prof_lo[4];
-          int hi = A.prof_hi[4];
-          if (qc == 0) { lo = A.prof_lo[0]; hi = A.prof_hi[0]; }
-          if (qc == 1) { lo = A.prof_lo[1]; hi = A.prof_hi[1]; }
-          if (qc == 2) { lo = A.prof_lo[2]; hi = A.prof_hi[2]; }
-          if (qc == 3) { lo = A.prof_lo[3]; hi = A.prof_hi[3]; }
+          if (qc == 0) lo = A.prof_lo[0];
+          if (qc == 1) lo = A.prof_lo[1];
+          if (qc == 2) lo = A.prof_lo[2];
+          if (qc == 3) lo = A.prof_lo[3];
           plo[c] = lo;
-          phi[c] = hi;
-          kE[c] = j * e_ins;
+          qpk[c / 4] |= (qc > 4 ? 4u : qc) << (8 * (c % 4));
           // first row (ksw.c:395-397): eh[0].h = h0, eh[j].h = max(h0-oe_ins-(j-1)e_ins, 0)
           const int g = j == 0 ? h0 : h0 - oe_ins - (j - 1) * e_ins;
           G[c] = g > 0 ? g : 0;
@@ -179,16 +218,14 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
         i = 0; beg = 0; end = qlen;
         best = h0; best_i = best_j = -1; gbest = -1; gbest_i = -1; moff = 0;
         ncell = 0;
-        prefetch();
+        tbase = tlen > 0 ? A.target + toff : A.query + nx_qoff;
+        load_window(tbase, tlen, 4 * L, tnx);
+        stage1();
+        load_desc();
         if (tlen <= 0) {
-          if (lig == 0) {
-            if (A.score) A.score[task] = h0;
-            if (A.qle) A.qle[task] = 0;
-            if (A.tle) A.tle[task] = 0;
-            if (A.gtle) A.gtle[task] = 0;
-            if (A.gscore) A.gscore[task] = -1;
-            if (A.max_off) A.max_off[task] = 0;
-            if (A.cells) A.cells[task] = 0;
+          if (lig < 8) {  // ksw.c:473-478 with no row: qle = tle = gtle = 0, gscore = -1
+            const int v = lig == 0 ? h0 : lig == 4 ? -1 : 0;
+            A.rec[(int64_t)task * 8 + lig] = v;
           }
         } else {
           need = false;
@@ -199,15 +236,13 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
     if (need || done) continue;
 
     // ---------------- one DP row (ksw.c:412-470) ----------------
-    if (i - tcb >= 4 * L) {  // next window of 4*L target bases
+    if (i - tcb >= 4 * L) {  // move to the prefetched window, prefetch the one after
       tcb = i;
       uint32_t v = 0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int r = i + 4 * lig + k;
-        v |= (r < tlen ? (uint32_t)A.target[toff + r] : 0u) << (8 * k);
-      }
+      for (int k2 = 0; k2 < 4; ++k2) v |= (i + 4 * lig + k2 < tlen ? (tnx[k2] & 0xffu) : 0u) << (8 * k2);
       tch = v;
+      load_window(tbase, tlen, i + 4 * L, tnx);
     }
     const int tr = i - tcb;
     const int t = (int)((__shfl(tch, tr >> 2, L) >> (8 * (tr & 3))) & 0xffu);
@@ -229,15 +264,31 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
     }
 
     int M[C], ex[C];
+    // substitution scores of this row; a target base N (code 4, which the
+    // 2-bit reference never produces) takes the slow path, decided per wave
+#pragma unroll
+    for (int c = 0; c < C; ++c) M[c] = sbyte(plo[c], t & 3);
+    if (__builtin_expect(__any(tN), 0)) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const uint32_t qc = (qpk[c / 4] >> (8 * (c % 4))) & 0xffu;
+        int sN = A.prof_hi[4];
+        if (qc == 0) sN = A.prof_hi[0];
+        if (qc == 1) sN = A.prof_hi[1];
+        if (qc == 2) sN = A.prof_hi[2];
+        if (qc == 3) sN = A.prof_hi[3];
+        M[c] = tN ? sN : M[c];
+      }
+    }
     int run = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int s = tN ? phi[c] : sbyte(plo[c], t & 3);
+      const int s = M[c];
       const int g = G[c];
       const int mm = g ? g + s : 0;
       M[c] = mm;
       // key + e_ins = max(M - oe_ins, 0) + (j+1)*e_ins  (>= 1)
-      const int key = max(mm - oe_ins, 0) + kE[c] + e_ins;
+      const int key = max(mm - oe_ins, 0) + kE0 + (c + 1) * e_ins;
       ex[c] = run;
       run = max(run, key);
     }
@@ -250,7 +301,7 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
       const int j = base + c;
       const bool live = j < end;
       const int x = max(ex[c], lex);
-      const int f = x - kE[c];  // F(j); 0 when no column precedes j
+      const int f = x - (kE0 + c * e_ins);  // F(j); 0 when no column precedes j
       const int hh = max(max(M[c], E[c]), f);
       h[c] = hh;
       const int ne = max(max(E[c] - e_del, M[c] - oe_del), 0);
@@ -310,14 +361,16 @@ ext_plain_kernel(ExtArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
       if (i >= tlen) fin = true;
     }
     if (fin) {
-      if (lig == 0) {
-        if (A.score) A.score[task] = best;
-        if (A.qle) A.qle[task] = best_j + 1;
-        if (A.tle) A.tle[task] = best_i + 1;
-        if (A.gtle) A.gtle[task] = gbest_i + 1;
-        if (A.gscore) A.gscore[task] = gbest;
-        if (A.max_off) A.max_off[task] = moff;
-        if (A.cells) A.cells[task] = (int64_t)ncell;
+      if (lig < 8) {  // ksw.c:473-478, one field per lane
+        int v = best;
+        v = lig == 1 ? best_j + 1 : v;
+        v = lig == 2 ? best_i + 1 : v;
+        v = lig == 3 ? gbest_i + 1 : v;
+        v = lig == 4 ? gbest : v;
+        v = lig == 5 ? moff : v;
+        v = lig == 6 ? (int)ncell : v;
+        v = lig == 7 ? 0 : v;
+        A.rec[(int64_t)task * 8 + lig] = v;
       }
       need = true;
     }
@@ -412,16 +465,47 @@ __global__ void ext_reset_kernel(ExtWork W) {
   if (k == 0) { W.err[0] = 0; W.err[1] = 0x7fffffff; }
 }
 
-template <int B>
-static void launch_bin(const ExtArgs& a, const ExtWork& wk, hipStream_t st, int64_t hint) {
+// Copies the packed per-task records into the caller's (nullable) arrays.
+__global__ void __launch_bounds__(256)
+ext_unpack_kernel(ExtArgs A, ExtWork W, const int32_t* __restrict__ rec) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (W.bin_of[t] == 0xff) continue;  // rejected: result slots stay untouched
+    const int4 a = *(const int4*)(rec + t * 8);
+    const int4 b = *(const int4*)(rec + t * 8 + 4);
+    if (A.score) A.score[t] = a.x;
+    if (A.qle) A.qle[t] = a.y;
+    if (A.tle) A.tle[t] = a.z;
+    if (A.gtle) A.gtle[t] = a.w;
+    if (A.gscore) A.gscore[t] = b.x;
+    if (A.max_off) A.max_off[t] = b.y;
+    if (A.cells) A.cells[t] = (int64_t)(uint32_t)b.z;
+  }
+}
+
+template <int B, bool E1>
+static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, hipStream_t st, int64_t hint) {
   constexpr int L = kExtShape[B].lanes, C = kExtShape[B].cols;
   // one block per kExtTasksPerBlock descriptors of the bin; without a host-side
   // count the grid covers the whole batch and surplus blocks exit at once
-  const int64_t n = hint >= 0 ? hint : a.n;
+  const int64_t n = hint >= 0 ? hint : n_all;
   const int64_t blocks = (n + kExtTasksPerBlock - 1) / kExtTasksPerBlock;
   if (blocks < 1) return;
-  hipLaunchKernelGGL((ext_plain_kernel<L, C>), dim3((unsigned)blocks), dim3(256), 0, st, a,
+  hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(256), 0, st, k,
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
+}
+
+template <bool E1>
+static void launch_bins(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st, const int64_t* hc) {
+  // longest queries first so the long tasks do not form the tail
+  launch_bin<7, E1>(k, wk, n, st, hc ? hc[7] : -1);
+  launch_bin<6, E1>(k, wk, n, st, hc ? hc[6] : -1);
+  launch_bin<5, E1>(k, wk, n, st, hc ? hc[5] : -1);
+  launch_bin<4, E1>(k, wk, n, st, hc ? hc[4] : -1);
+  launch_bin<3, E1>(k, wk, n, st, hc ? hc[3] : -1);
+  launch_bin<2, E1>(k, wk, n, st, hc ? hc[2] : -1);
+  launch_bin<1, E1>(k, wk, n, st, hc ? hc[1] : -1);
+  launch_bin<0, E1>(k, wk, n, st, hc ? hc[0] : -1);
 }
 
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
@@ -433,10 +517,13 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk);
   const int64_t per_block = (a.n + blocks - 1) / blocks;
   hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
-  // longest queries first so the long tasks do not form the tail
-#define BSX_BIN(B) launch_bin<B>(a, wk, st, host_bin_counts ? host_bin_counts[B] : -1)
-  BSX_BIN(7); BSX_BIN(6); BSX_BIN(5); BSX_BIN(4); BSX_BIN(3); BSX_BIN(2); BSX_BIN(1); BSX_BIN(0);
-#undef BSX_BIN
+  ExtKArgs k{};
+  k.query = a.query; k.target = a.target; k.rec = wk.rec;
+  k.o_del = a.o_del; k.e_del = a.e_del; k.o_ins = a.o_ins; k.e_ins = a.e_ins; k.zdrop = a.zdrop;
+  for (int q = 0; q < 5; ++q) { k.prof_lo[q] = a.prof_lo[q]; k.prof_hi[q] = a.prof_hi[q]; }
+  if (a.e_del == 1 && a.e_ins == 1) launch_bins<true>(k, wk, a.n, st, host_bin_counts);
+  else launch_bins<false>(k, wk, a.n, st, host_bin_counts);
+  hipLaunchKernelGGL(ext_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, (const int32_t*)wk.rec);
   return hipGetLastError();
 }
 
