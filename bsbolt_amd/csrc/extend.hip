@@ -36,59 +36,11 @@
 // per iteration) so the groups of a wave never wait for one another between
 // tasks.
 #include "bsx_internal.h"
+#include "ext_common.h"
 
 namespace bsx {
 
-// ---- DPP group primitives (identity 0, values >= 0) ----
-template <int L>
-__device__ __forceinline__ int grp_incl_max(int x) {
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true));  // row_shr:1
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true));  // row_shr:2
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true));  // row_shr:4
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true));  // row_shr:8
-  if (L >= 32) x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
-  if (L >= 64) x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
-  return x;
-}
 
-// exclusive version of an inclusive scan result (lane 0 of the group gets 0)
-template <int L>
-__device__ __forceinline__ int grp_excl_from_incl(int incl, int lig) {
-  if (L == 16) return __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
-  const int e = __builtin_amdgcn_update_dpp(0, incl, 0x138, 0xf, 0xf, true);        // wave_shr:1
-  return lig == 0 ? 0 : e;
-}
-
-// every lane of the group receives the maximum
-template <int L>
-__device__ __forceinline__ int grp_max(int x, int lane) {
-  if (L == 16) {
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true));   // quad_perm [1,0,3,2]
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true));   // quad_perm [2,3,0,1]
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true));  // row_half_mirror
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true));  // row_mirror
-    return x;
-  }
-  x = grp_incl_max<L>(x);
-  if (L == 64) return __builtin_amdgcn_readlane(x, 63);
-  const int a = __builtin_amdgcn_readlane(x, 31), b = __builtin_amdgcn_readlane(x, 63);
-  return lane < 32 ? a : b;
-}
-
-__device__ __forceinline__ int sbyte(uint32_t w, int t) {  // signed byte t of w
-  return (int)((int32_t)(w << (24 - 8 * t)) >> 24);
-}
-
-// Kernel-side arguments of the extension kernels: only what the row loop and
-// the task start read (the output arrays are written by ext_unpack_kernel).
-struct ExtKArgs {
-  const uint8_t* query;
-  const uint8_t* target;
-  int32_t* rec;  // [n][8]: score qle tle gtle gscore max_off cells_lo cells_hi
-  int o_del, e_del, o_ins, e_ins, zdrop;
-  uint32_t prof_lo[5];
-  int32_t prof_hi[5];
-};
 
 template <int L, int C, bool E1>
 __global__ void __launch_bounds__(256)
