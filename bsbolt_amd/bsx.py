@@ -30,7 +30,18 @@ _ERRNAME = {BSX_EINVAL: "BSX_EINVAL", BSX_EDEVICE: "BSX_EDEVICE", BSX_ENOMEM: "B
 EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx_sync", "bsx_version",
            "bsx_default_scoring", "bsx_extend_batch", "bsx_extend_batch_device", "bsx_global_batch",
            "bsx_global_batch_device", "bsx_set_global_zslot", "bsx_ksw_extend2", "bsx_dev_alloc",
-           "bsx_dev_free", "bsx_memcpy", "bsx_timer_start", "bsx_timer_stop")
+           "bsx_dev_free", "bsx_memcpy", "bsx_timer_start", "bsx_timer_stop", "bsx_set_band_policy",
+           "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get", "bsx_band_features_device",
+           "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
+           "bsx_band_train_device")
+
+# learned-band form (include/bsx.h)
+BAND_PLAIN, BAND_LEARNED, BAND_FIXED = 0, 1, 2
+BAND_CLASSES = 5              # classes 0..3 = w 8/16/32/64, 4 = the task's own band
+BAND_WIDTHS = (8, 16, 32, 64)
+BAND_FEATURES = 24
+BAND_FEATURE_STRIDE = 32
+BAND_PARAMS = 6085
 
 
 class BsxError(RuntimeError):
@@ -110,6 +121,19 @@ def lib():
         L.bsx_timer_start.argtypes = [C.c_void_p]
         L.bsx_timer_stop.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         P = C.c_void_p
+        L.bsx_set_band_policy.argtypes = [P, C.c_int, C.c_int]
+        L.bsx_band_stats.argtypes = [P, P]
+        L.bsx_band_model_set.argtypes = [P, P, C.c_int64]
+        L.bsx_band_model_get.argtypes = [P, P, C.c_int64]
+        L.bsx_band_features_device.argtypes = [P, C.POINTER(Scoring), C.POINTER(_ExtTasks), P]
+        L.bsx_band_labels_device.argtypes = [P, C.POINTER(Scoring), C.POINTER(_ExtTasks), P]
+        L.bsx_band_forward_device.argtypes = [P, P, C.c_int64, P, P]
+        L.bsx_band_grad_device.argtypes = [P, P, P, C.c_int64, P]
+        L.bsx_band_train_device.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int, C.c_float, P]
+        for f in ("bsx_set_band_policy", "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get",
+                  "bsx_band_features_device", "bsx_band_labels_device", "bsx_band_forward_device",
+                  "bsx_band_grad_device", "bsx_band_train_device"):
+            getattr(L, f).restype = C.c_int
         L.bsx_ksw_extend2.restype = C.c_int
         L.bsx_ksw_extend2.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P] + [C.c_int] * 8 + [P] * 5
         _lib = L
@@ -219,6 +243,84 @@ class Engine:
                                     "end_bonus")])
         rs = _ExtResults(*[dev.get(k) for k in RESULT_FIELDS + ("cells",)])
         self._check(self.L.bsx_extend_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
+
+    # ---- learned-band form ----
+    def set_band_policy(self, policy, band_class=0):
+        self._check(self.L.bsx_set_band_policy(self.ctx, int(policy), int(band_class)))
+
+    def band_stats(self):
+        """(routed, rejected) per class of the last learned/fixed extension call."""
+        out = np.zeros(2 * BAND_CLASSES, np.int64)
+        self._check(self.L.bsx_band_stats(self.ctx, out.ctypes.data))
+        return out.reshape(BAND_CLASSES, 2)
+
+    def set_band_model(self, params):
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        self._check(self.L.bsx_band_model_set(self.ctx, p.ctypes.data, p.size))
+
+    def get_band_model(self):
+        p = np.zeros(BAND_PARAMS, np.float32)
+        self._check(self.L.bsx_band_model_get(self.ctx, p.ctypes.data, p.size))
+        return p
+
+    def _ext_tasks_dev(self, dev):
+        return _ExtTasks(dev["n"], *[dev.get(k) for k in
+                                     ("query", "q_off", "qlen", "target", "t_off", "tlen", "h0", "w",
+                                      "end_bonus")])
+
+    def band_features_device(self, dev, feats_ptr, scoring: Scoring = None):
+        sc = scoring or Scoring.make()
+        self._check(self.L.bsx_band_features_device(self.ctx, C.byref(sc), C.byref(self._ext_tasks_dev(dev)),
+                                                     feats_ptr))
+
+    def band_labels_device(self, dev, labels_ptr, scoring: Scoring = None):
+        sc = scoring or Scoring.make()
+        self._check(self.L.bsx_band_labels_device(self.ctx, C.byref(sc), C.byref(self._ext_tasks_dev(dev)),
+                                                   labels_ptr))
+
+    def band_features(self, tasks: ExtTasks, scoring: Scoring = None):
+        """Features of a host batch -> uint16 [n, 32] (BF16 bit patterns)."""
+        keep, dev = self.tasks_to_device(tasks)
+        dev["n"] = len(tasks)
+        f = self.empty((len(tasks), BAND_FEATURE_STRIDE), np.uint16)
+        self.band_features_device(dev, f.ptr, scoring)
+        self.sync()
+        return f.numpy()
+
+    def band_labels(self, tasks: ExtTasks, scoring: Scoring = None):
+        """Smallest band class the certificate accepts per task (4: none)."""
+        keep, dev = self.tasks_to_device(tasks)
+        dev["n"] = len(tasks)
+        lab = self.empty((len(tasks),), np.uint8)
+        self.band_labels_device(dev, lab.ptr, scoring)
+        self.sync()
+        return lab.numpy()
+
+    def band_forward(self, feats):
+        """MLP forward of host features uint16 [n, 32] -> (logits [n, 5], class [n])."""
+        feats = np.ascontiguousarray(feats, np.uint16)
+        n = feats.shape[0]
+        fd = self.to_device(feats)
+        lg = self.empty((n, BAND_CLASSES), np.float32)
+        cl = self.empty((n,), np.uint8)
+        self._check(self.L.bsx_band_forward_device(self.ctx, fd.ptr, n, lg.ptr, cl.ptr))
+        self.sync()
+        return lg.numpy(), cl.numpy()
+
+    def band_grad(self, feats, labels):
+        """Mean cross-entropy gradient (6085) and mean loss over host samples."""
+        feats = np.ascontiguousarray(feats, np.uint16)
+        labels = np.ascontiguousarray(labels, np.uint8)
+        fd, ld = self.to_device(feats), self.to_device(labels)
+        g = self.empty((BAND_PARAMS + 1,), np.float32)
+        self._check(self.L.bsx_band_grad_device(self.ctx, fd.ptr, ld.ptr, feats.shape[0], g.ptr))
+        self.sync()
+        out = g.numpy()
+        return out[:BAND_PARAMS], float(out[BAND_PARAMS])
+
+    def band_train_device(self, feats_ptr, labels_ptr, n, batch, steps, lr, losses_ptr=None):
+        self._check(self.L.bsx_band_train_device(self.ctx, feats_ptr, labels_ptr, int(n), int(batch), int(steps),
+                                                 float(lr), losses_ptr))
 
     # ---- global ----
     def global_(self, tasks: GlbTasks, scoring: Scoring = None, cigar=True, slot=None,

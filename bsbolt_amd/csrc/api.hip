@@ -12,11 +12,19 @@
 
 #include "../../include/bsx.h"
 #include "bsx_internal.h"
+#include "mlp.h"
 
 using namespace bsx;
 
-// counts[kExtBins], cursor[kExtBins], err[4], fetch[kExtBins * kExtSlices]
-static constexpr size_t kSmallBytes = 4 * (2 * kExtBins + 4 + kExtBins * kExtSlices);
+// counts[kCountSlots], cursor[kCountSlots], err[4], fetch[kExtBins * kExtSlices]
+static constexpr size_t kSmallBytes = 4 * (2 * kCountSlots + 4 + kExtBins * kExtSlices);
+
+// band model buffer: params, adam_m, adam_v (FP32), then the BF16 operand copies
+static constexpr size_t kAlign = 256;
+static constexpr size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+static constexpr size_t kParamBytes = align_up(4 * (size_t)kParamCount);
+static constexpr size_t kModelBytes = 3 * kParamBytes + align_up(2 * 64 * kFeatPad) + 2 * align_up(2 * 64 * 64) +
+                                      2 * align_up(2 * 16 * 64);
 
 namespace {
 
@@ -51,6 +59,13 @@ struct bsx_ctx {
   std::string err;
   // extension workspace
   DevBuf bin_of, list, desc, rec, small;  // small: counts/cursor/err/fetch
+  // learned-band form
+  DevBuf nbin_of, need_plain, cls, stats, model, part, grad;
+  int band_policy = BSX_BAND_PLAIN;
+  int band_class = 0;
+  bool model_ready = false;
+  int adam_step = 0;
+  int64_t last_n = 0;  // tasks of the last extension call (for bsx_band_stats)
   DevBuf zbuf;                 // direction-byte slots of the global kernel
   int64_t zslot = 1 << 16;     // bytes per slot: largest min(qlen,2w+1)*tlen accepted
   // staging for the host-memory entry points
@@ -149,7 +164,8 @@ void bsx_destroy(bsx_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->bin_of, &c->list, &c->desc, &c->rec, &c->small, &c->zbuf, &c->h_query, &c->h_target, &c->h_qoff, &c->h_toff,
-                    &c->h_i32, &c->h_res})
+                    &c->h_i32, &c->h_res, &c->nbin_of, &c->need_plain, &c->cls, &c->stats, &c->model, &c->part,
+                    &c->grad})
     b->release();
   if (c->err_host) (void)hipHostFree(c->err_host);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -181,17 +197,43 @@ int bsx_sync(bsx_ctx* c) {
   return BSX_OK;
 }
 
-int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t,
-                            bsx_ext_results* r) {
-  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+static MlpDev model_view(bsx_ctx* c) {
+  char* b = (char*)c->model.p;
+  MlpDev m;
+  m.params = (float*)b;
+  m.adam_m = (float*)(b + kParamBytes);
+  m.adam_v = (float*)(b + 2 * kParamBytes);
+  size_t o = 3 * kParamBytes;
+  m.w1p = (uint16_t*)(b + o); o += align_up(2 * 64 * kFeatPad);
+  m.w2p = (uint16_t*)(b + o); o += align_up(2 * 64 * 64);
+  m.w2t = (uint16_t*)(b + o); o += align_up(2 * 64 * 64);
+  m.w3p = (uint16_t*)(b + o); o += align_up(2 * 16 * 64);
+  m.w3t = (uint16_t*)(b + o);
+  return m;
+}
+
+static ExtWork ext_work(bsx_ctx* c) {
+  ExtWork wk;
+  wk.bin_of = (uint8_t*)c->bin_of.p;
+  wk.desc = (ExtDesc*)c->desc.p;
+  wk.rec = (int32_t*)c->rec.p;
+  wk.counts = (uint32_t*)c->small.p;
+  wk.cursor = wk.counts + kCountSlots;
+  wk.err = (int32_t*)(wk.cursor + kCountSlots);
+  wk.fetch = (uint32_t*)(wk.err + 4);
+  wk.nbin_of = (uint8_t*)c->nbin_of.p;
+  wk.need_plain = (uint8_t*)c->need_plain.p;
+  return wk;
+}
+
+static int ext_args(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, ExtArgs& a) {
   int rc = check_scoring(c, sc);
   if (rc) return rc;
   if (t->n < 0 || t->n > INT_MAX) return fail(c, BSX_EINVAL, "n must be in [0, 2^31-1] (got %lld)", (long long)t->n);
   if (t->n > 0 && (!t->query || !t->q_off || !t->qlen || !t->target || !t->t_off || !t->tlen || !t->h0 ||
                    !t->w || !t->end_bonus))
     return fail(c, BSX_EINVAL, "task arrays must not be NULL");
-  (void)hipSetDevice(c->device);
-  ExtArgs a{};
+  a = ExtArgs{};
   a.o_del = sc->o_del; a.e_del = sc->e_del; a.o_ins = sc->o_ins; a.e_ins = sc->e_ins;
   a.zdrop = sc->zdrop;
   a.mat_max = mat_max(sc);
@@ -200,26 +242,181 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   a.query = t->query; a.q_off = t->q_off; a.qlen = t->qlen;
   a.target = t->target; a.t_off = t->t_off; a.tlen = t->tlen;
   a.h0 = t->h0; a.w = t->w; a.end_bonus = t->end_bonus;
-  a.score = r->score; a.qle = r->qle; a.tle = r->tle; a.gtle = r->gtle; a.gscore = r->gscore;
-  a.max_off = r->max_off; a.cells = r->cells;
-  const size_t n = (size_t)t->n;
+  return BSX_OK;
+}
+
+static int ext_buffers(bsx_ctx* c, size_t n) {
   HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
   HIPCHK(c, c->desc.ensure(sizeof(ExtDesc) * (n + 1)), "hipMalloc(desc)");
   HIPCHK(c, c->rec.ensure(32 * (n + 1)), "hipMalloc(rec)");
   HIPCHK(c, c->small.ensure(kSmallBytes), "hipMalloc(work)");
-  ExtWork wk;
-  wk.bin_of = (uint8_t*)c->bin_of.p;
-  wk.desc = (ExtDesc*)c->desc.p;
-  wk.rec = (int32_t*)c->rec.p;
-  wk.counts = (uint32_t*)c->small.p;
-  wk.cursor = wk.counts + kExtBins;
-  wk.err = (int32_t*)(wk.cursor + kExtBins);
-  wk.fetch = (uint32_t*)(wk.err + 4);
-  HIPCHK(c, launch_extend(a, wk, c->num_cu, c->stream, nullptr), "extension launch");
+  HIPCHK(c, c->nbin_of.ensure(n + 1), "hipMalloc(nbin_of)");
+  HIPCHK(c, c->need_plain.ensure(n + 1), "hipMalloc(need_plain)");
+  return BSX_OK;
+}
+
+int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t,
+                            bsx_ext_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  ExtArgs a;
+  int rc = ext_args(c, sc, t, a);
+  if (rc) return rc;
+  a.score = r->score; a.qle = r->qle; a.tle = r->tle; a.gtle = r->gtle; a.gscore = r->gscore;
+  a.max_off = r->max_off; a.cells = r->cells;
+  (void)hipSetDevice(c->device);
+  const size_t n = (size_t)t->n;
+  rc = ext_buffers(c, n);
+  if (rc) return rc;
+  ExtWork wk = ext_work(c);
+  BandPlan plan{kBandPlain, 0, nullptr, false, false};
+  if (c->band_policy == BSX_BAND_LEARNED) {
+    if (!c->model_ready) return fail(c, BSX_EINVAL, "learned band policy without a band model (bsx_band_model_set)");
+    HIPCHK(c, c->cls.ensure(n + 1), "hipMalloc(cls)");
+    HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), (uint8_t*)c->cls.p, c->num_cu, c->stream), "band predict");
+    plan = BandPlan{kBandLearned, 0, (const uint8_t*)c->cls.p, false, false};
+  } else if (c->band_policy == BSX_BAND_FIXED) {
+    plan = BandPlan{kBandFixed, c->band_class, nullptr, true, false};
+  }
+  HIPCHK(c, launch_extend(a, wk, c->num_cu, c->stream, nullptr, &plan), "extension launch");
   HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
          "hipMemcpyAsync(err)");
+  c->last_n = plan.mode == kBandPlain ? 0 : (int64_t)n;
   c->pending_check = true;
   c->pending_kind = 1;
+  return BSX_OK;
+}
+
+int bsx_set_band_policy(bsx_ctx* c, int policy, int band_class) {
+  if (!c) return BSX_EINVAL;
+  if (policy != BSX_BAND_PLAIN && policy != BSX_BAND_LEARNED && policy != BSX_BAND_FIXED)
+    return fail(c, BSX_EINVAL, "unknown band policy %d", policy);
+  if (policy == BSX_BAND_FIXED && (band_class < 0 || band_class >= kBandClasses - 1))
+    return fail(c, BSX_EINVAL, "fixed band class must be in [0, %d] (got %d)", kBandClasses - 2, band_class);
+  c->band_policy = policy;
+  c->band_class = band_class;
+  return BSX_OK;
+}
+
+int bsx_band_stats(bsx_ctx* c, int64_t* out) {
+  if (!c || !out) return fail(c, BSX_EINVAL, "NULL argument");
+  if (c->last_n <= 0) {
+    for (int k = 0; k < 2 * kBandClasses; ++k) out[k] = 0;
+    return BSX_OK;
+  }
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, c->stats.ensure(2 * kBandClasses * sizeof(unsigned long long)), "hipMalloc(stats)");
+  ExtWork wk = ext_work(c);
+  HIPCHK(c, launch_band_stats(c->last_n, wk, (unsigned long long*)c->stats.p, c->num_cu, c->stream), "band stats");
+  unsigned long long h[2 * kBandClasses];
+  HIPCHK(c, hipMemcpyAsync(h, c->stats.p, sizeof h, hipMemcpyDeviceToHost, c->stream), "D2H stats");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (int k = 0; k < 2 * kBandClasses; ++k) out[k] = (int64_t)h[k];
+  return BSX_OK;
+}
+
+int bsx_band_model_set(bsx_ctx* c, const float* params, int64_t n) {
+  if (!c || !params) return fail(c, BSX_EINVAL, "NULL argument");
+  if (n != kParamCount) return fail(c, BSX_EINVAL, "band model has %d parameters (got %lld)", kParamCount, (long long)n);
+  for (int64_t k = 0; k < n; ++k)
+    if (!(params[k] > -1e30f && params[k] < 1e30f)) return fail(c, BSX_EINVAL, "parameter %lld is not finite", (long long)k);
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, c->model.ensure(kModelBytes), "hipMalloc(model)");
+  MlpDev m = model_view(c);
+  HIPCHK(c, hipMemcpyAsync(m.params, params, 4 * (size_t)kParamCount, hipMemcpyHostToDevice, c->stream), "H2D model");
+  std::vector<float> zero(kParamCount, 0.f);
+  HIPCHK(c, hipMemcpyAsync(m.adam_m, zero.data(), 4 * (size_t)kParamCount, hipMemcpyHostToDevice, c->stream), "H2D adam");
+  HIPCHK(c, hipMemcpyAsync(m.adam_v, zero.data(), 4 * (size_t)kParamCount, hipMemcpyHostToDevice, c->stream), "H2D adam");
+  HIPCHK(c, mlp_launch_pack(m, c->stream), "model pack");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  c->model_ready = true;
+  c->adam_step = 0;
+  return BSX_OK;
+}
+
+int bsx_band_model_get(bsx_ctx* c, float* params, int64_t n) {
+  if (!c || !params) return fail(c, BSX_EINVAL, "NULL argument");
+  if (n != kParamCount) return fail(c, BSX_EINVAL, "band model has %d parameters (got %lld)", kParamCount, (long long)n);
+  if (!c->model_ready) return fail(c, BSX_EINVAL, "no band model");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipMemcpyAsync(params, model_view(c).params, 4 * (size_t)kParamCount, hipMemcpyDeviceToHost, c->stream),
+         "D2H model");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return BSX_OK;
+}
+
+int bsx_band_features_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, uint16_t* feats) {
+  if (!c || !t || (t->n > 0 && !feats)) return fail(c, BSX_EINVAL, "NULL argument");
+  ExtArgs a;
+  int rc = ext_args(c, sc, t, a);
+  if (rc) return rc;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, mlp_launch_features(a, feats, c->num_cu, c->stream), "band features");
+  return BSX_OK;
+}
+
+int bsx_band_labels_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, uint8_t* labels) {
+  if (!c || !t || (t->n > 0 && !labels)) return fail(c, BSX_EINVAL, "NULL argument");
+  ExtArgs a;
+  int rc = ext_args(c, sc, t, a);
+  if (rc) return rc;
+  if (a.n == 0) return BSX_OK;
+  (void)hipSetDevice(c->device);
+  rc = ext_buffers(c, (size_t)a.n);
+  if (rc) return rc;
+  ExtWork wk = ext_work(c);
+  for (int cl = kBandClasses - 2; cl >= 0; --cl) {
+    BandPlan plan{kBandFixed, cl, nullptr, true, true};
+    HIPCHK(c, launch_extend(a, wk, c->num_cu, c->stream, nullptr, &plan), "narrow launch");
+    HIPCHK(c, launch_band_label_merge(a, wk, cl, labels, cl == kBandClasses - 2, c->num_cu, c->stream), "label merge");
+  }
+  HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+         "hipMemcpyAsync(err)");
+  c->last_n = 0;
+  c->pending_check = true;
+  c->pending_kind = 1;
+  return BSX_OK;
+}
+
+int bsx_band_forward_device(bsx_ctx* c, const uint16_t* feats, int64_t n, float* logits, uint8_t* cls) {
+  if (!c || (n > 0 && !feats) || n < 0) return fail(c, BSX_EINVAL, "bad argument");
+  if (!c->model_ready) return fail(c, BSX_EINVAL, "no band model");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, mlp_launch_forward(model_view(c), feats, n, logits, cls, c->num_cu, c->stream), "band forward");
+  return BSX_OK;
+}
+
+static int band_grad(bsx_ctx* c, const uint16_t* feats, const uint8_t* labels, int64_t n, float* grad) {
+  HIPCHK(c, c->part.ensure(4 * (size_t)(kParamCount + 1) * (size_t)mlp_grad_waves(n)), "hipMalloc(part)");
+  HIPCHK(c, mlp_launch_grad(model_view(c), feats, labels, n, (float*)c->part.p, grad, c->stream), "band grad");
+  return BSX_OK;
+}
+
+int bsx_band_grad_device(bsx_ctx* c, const uint16_t* feats, const uint8_t* labels, int64_t n, float* grad) {
+  if (!c || !grad || n < 1 || !feats || !labels) return fail(c, BSX_EINVAL, "bad argument");
+  if (!c->model_ready) return fail(c, BSX_EINVAL, "no band model");
+  (void)hipSetDevice(c->device);
+  return band_grad(c, feats, labels, n, grad);
+}
+
+int bsx_band_train_device(bsx_ctx* c, const uint16_t* feats, const uint8_t* labels, int64_t n, int64_t batch,
+                          int steps, float lr, float* losses) {
+  if (!c || n < 1 || !feats || !labels || batch < 1 || steps < 0 || !(lr > 0.f))
+    return fail(c, BSX_EINVAL, "bad argument");
+  if (!c->model_ready) return fail(c, BSX_EINVAL, "no band model");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, c->grad.ensure(4 * (size_t)(kParamCount + 1)), "hipMalloc(grad)");
+  float* g = (float*)c->grad.p;
+  int64_t start = 0;
+  for (int s = 0; s < steps; ++s) {
+    if (start >= n) start = 0;
+    const int64_t nb = batch < n - start ? batch : n - start;
+    int rc = band_grad(c, feats + start * kFeatPad, labels + start, nb, g);
+    if (rc) return rc;
+    HIPCHK(c, mlp_launch_adam(model_view(c), g, lr, ++c->adam_step, c->stream), "adam");
+    if (losses)
+      HIPCHK(c, hipMemcpyAsync(losses + s, g + kParamCount, 4, hipMemcpyDeviceToDevice, c->stream), "loss copy");
+    start += nb;
+  }
   return BSX_OK;
 }
 
@@ -383,8 +580,8 @@ int bsx_global_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tas
   wk.bin_of = (uint8_t*)c->bin_of.p;
   wk.list = (int32_t*)c->list.p;
   wk.counts = (uint32_t*)c->small.p;
-  wk.cursor = wk.counts + kExtBins;
-  wk.err = (int32_t*)(wk.cursor + kExtBins);
+  wk.cursor = wk.counts + kCountSlots;
+  wk.err = (int32_t*)(wk.cursor + kCountSlots);
   wk.fetch = (uint32_t*)(wk.err + 4);
   wk.zslot = c->zslot;
   wk.z = nullptr;

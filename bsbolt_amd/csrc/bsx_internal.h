@@ -16,6 +16,21 @@ constexpr int kExtSlices = 8;           // fetch-counter slots per bin (CIGAR pa
 constexpr int kExtTasksPerBlock = 128;  // task descriptors owned by one extension block
 constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
 
+// Learned-band form: band classes 0..3 run the narrow kernel at w = 8, 16, 32,
+// 64; class 4 is the task's own band (plain kernel).  Count slots: plain bins
+// [0, kExtBins), narrow classes [kExtBins, kExtBins + kBandClasses - 1).
+constexpr int kBandClasses = 5;
+constexpr int kCountSlots = 16;
+constexpr int kNarrowQMax = 256;  // longest query the narrow kernels take
+enum BandMode { kBandPlain = 0, kBandLearned = 1, kBandFixed = 2 };
+struct BandPlan {
+  int mode;            // BandMode
+  int fixed;           // class for kBandFixed
+  const uint8_t* cls;  // [n] predicted class for kBandLearned
+  bool any_gain;       // route a task to its class even where the frame is not narrower than its bin
+  bool skip_plain;     // narrow pass only (labelling): no plain pass, no result unpack
+};
+
 // Scores are tracked as (h << 11 | column) keys in 32-bit lanes.
 constexpr int kScoreLimit = 1 << 20;
 
@@ -53,14 +68,19 @@ struct ExtWork {
   uint8_t* bin_of;     // [n] bin id, 0xff = rejected task
   ExtDesc* desc;       // [n] task descriptors grouped by bin
   int32_t* rec;        // [n][8] packed per-task results
-  uint32_t* counts;    // [kExtBins] tasks per bin
-  uint32_t* cursor;    // [kExtBins] scatter cursors
+  uint32_t* counts;    // [kCountSlots] tasks per bin / narrow class
+  uint32_t* cursor;    // [kCountSlots] scatter cursors
   uint32_t* fetch;     // [kExtBins * kExtSlices] persistent-kernel fetch counters
   int32_t* err;        // [0] rejected count, [1] lowest rejected index
+  uint8_t* nbin_of;    // [n] narrow class of the task, 0xff = plain only (learned-band form)
+  uint8_t* need_plain; // [n] 1 = the plain kernel computes the task (learned-band form)
 };
 
+struct ExtKArgs;
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
-                         const int64_t* host_bin_counts);
+                         const int64_t* host_bin_counts, const BandPlan* plan);
+hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st,
+                         const BandPlan& plan);
 
 struct GlbArgs {
   int o_del, e_del, o_ins, e_ins;

@@ -56,4 +56,25 @@ struct ExtKArgs {
   int32_t prof_hi[5];
 };
 
+// Band of a task clamped as ksw.c:399-407, in the same double arithmetic.
+__device__ __forceinline__ int ext_clamped_w(const ExtArgs& A, int64_t t) {
+  const int ql = A.qlen[t], eb = A.end_bonus[t];
+  int w = A.w[t];
+  int lim = (int)((double)(ql * A.mat_max + eb - A.o_ins) / A.e_ins + 1.);
+  lim = lim > 1 ? lim : 1;
+  w = w < lim ? w : lim;
+  lim = (int)((double)(ql * A.mat_max + eb - A.o_del) / A.e_del + 1.);
+  lim = lim > 1 ? lim : 1;
+  return w < lim ? w : lim;
+}
+
+// 48-byte work-list descriptor of task t (clamped band included).
+__device__ __forceinline__ void write_desc(const ExtArgs& A, int64_t t, ExtDesc* dst) {
+  const uint64_t qo = A.q_off[t], to = A.t_off[t];
+  int4* d = (int4*)dst;
+  d[0] = make_int4((int)t, A.qlen[t], A.tlen[t], A.h0[t]);
+  d[1] = make_int4(ext_clamped_w(A, t), 0, (int)(uint32_t)qo, (int)(uint32_t)(qo >> 32));
+  d[2] = make_int4((int)(uint32_t)to, (int)(uint32_t)(to >> 32), 0, 0);
+}
+
 }  // namespace bsx

@@ -338,14 +338,22 @@ __device__ __forceinline__ int ext_bin_of(int qlen) {
   return -1;
 }
 
+// mode 0: validate every task, write bin_of and count per bin
+// mode 1: validate only (the learned-band form counts after its narrow pass)
+// mode 2: count the valid tasks still flagged in need[] (no validation)
 __global__ void __launch_bounds__(256)
-ext_bin_count_kernel(ExtArgs A, ExtWork W) {
+ext_bin_count_kernel(ExtArgs A, ExtWork W, int mode) {
   __shared__ uint32_t hist[kExtBins];
   if (threadIdx.x < kExtBins) hist[threadIdx.x] = 0;
   __syncthreads();
   int bad = 0, first_bad = 0x7fffffff;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
+    if (mode == 2) {
+      const int b = W.bin_of[t];
+      if (b != 0xff && W.need_plain[t]) atomicAdd(&hist[b], 1u);
+      continue;
+    }
     const int ql = A.qlen[t], tl = A.tlen[t], h0 = A.h0[t], w = A.w[t], eb = A.end_bonus[t];
     int b = ext_bin_of(ql);
     const bool ok = ql >= 1 && tl >= 0 && h0 >= 1 && w >= 0 && b >= 0 && eb > -kScoreLimit &&
@@ -354,7 +362,7 @@ ext_bin_count_kernel(ExtArgs A, ExtWork W) {
       b = 0xff;
       ++bad;
       first_bad = min(first_bad, (int)t);
-    } else {
+    } else if (mode == 0) {
       atomicAdd(&hist[b], 1u);
     }
     W.bin_of[t] = (uint8_t)b;
@@ -371,7 +379,7 @@ ext_bin_count_kernel(ExtArgs A, ExtWork W) {
 // LDS, reserves one range per bin with a single global atomic, and writes the
 // descriptors of its tasks into those ranges.
 __global__ void __launch_bounds__(256)
-ext_bin_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
+ext_bin_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, const uint8_t* __restrict__ need) {
   __shared__ uint32_t off[kExtBins], hist[kExtBins], cur[kExtBins];
   const int64_t t0 = (int64_t)blockIdx.x * per_block;
   const int64_t t1 = min(A.n, t0 + per_block);
@@ -379,7 +387,7 @@ ext_bin_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
   __syncthreads();
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int b = W.bin_of[t];
-    if (b != 0xff) atomicAdd(&hist[b], 1u);
+    if (b != 0xff && (!need || need[t])) atomicAdd(&hist[b], 1u);
   }
   __syncthreads();
   if (threadIdx.x < kExtBins) {
@@ -391,28 +399,15 @@ ext_bin_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
   __syncthreads();
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int b = W.bin_of[t];
-    if (b == 0xff) continue;
+    if (b == 0xff || (need && !need[t])) continue;
     const uint32_t p = off[b] + atomicAdd(&cur[b], 1u);
-    const int ql = A.qlen[t], eb = A.end_bonus[t];
-    int w = A.w[t];
-    // band clamp of ksw.c:399-407, in the same double arithmetic
-    int lim = (int)((double)(ql * A.mat_max + eb - A.o_ins) / A.e_ins + 1.);
-    lim = lim > 1 ? lim : 1;
-    w = w < lim ? w : lim;
-    lim = (int)((double)(ql * A.mat_max + eb - A.o_del) / A.e_del + 1.);
-    lim = lim > 1 ? lim : 1;
-    w = w < lim ? w : lim;
-    const uint64_t qo = A.q_off[t], to = A.t_off[t];
-    int4* d = (int4*)(W.desc + p);
-    d[0] = make_int4((int)t, ql, A.tlen[t], A.h0[t]);
-    d[1] = make_int4(w, 0, (int)(uint32_t)qo, (int)(uint32_t)(qo >> 32));
-    d[2] = make_int4((int)(uint32_t)to, (int)(uint32_t)(to >> 32), 0, 0);
+    write_desc(A, t, W.desc + p);
   }
 }
 
 __global__ void ext_reset_kernel(ExtWork W) {
   const int k = threadIdx.x;
-  if (k < kExtBins) { W.counts[k] = 0; W.cursor[k] = 0; }
+  if (k < kCountSlots) { W.counts[k] = 0; W.cursor[k] = 0; }
   if (k < kExtBins * kExtSlices) W.fetch[k] = 0;
   if (k == 0) { W.err[0] = 0; W.err[1] = 0x7fffffff; }
 }
@@ -461,18 +456,26 @@ static void launch_bins(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStre
 }
 
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
-                         const int64_t* host_bin_counts) {
+                         const int64_t* host_bin_counts, const BandPlan* plan) {
   hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(kExtBins * kExtSlices), 0, st, wk);
   if (a.n <= 0) return hipGetLastError();
   int64_t blocks = (a.n + 255) / 256;
   if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
-  hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk);
-  const int64_t per_block = (a.n + blocks - 1) / blocks;
-  hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
+  const bool band = plan && plan->mode != kBandPlain;
   ExtKArgs k{};
   k.query = a.query; k.target = a.target; k.rec = wk.rec;
   k.o_del = a.o_del; k.e_del = a.e_del; k.o_ins = a.o_ins; k.e_ins = a.e_ins; k.zdrop = a.zdrop;
   for (int q = 0; q < 5; ++q) { k.prof_lo[q] = a.prof_lo[q]; k.prof_hi[q] = a.prof_hi[q]; }
+  hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, band ? 1 : 0);
+  if (band) {
+    hipError_t e = launch_narrow(a, k, wk, num_cu, st, *plan);
+    if (e != hipSuccess || plan->skip_plain) return e;
+    hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, 2);
+    host_bin_counts = nullptr;
+  }
+  const int64_t per_block = (a.n + blocks - 1) / blocks;
+  hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block,
+                     band ? (const uint8_t*)wk.need_plain : nullptr);
   if (a.e_del == 1 && a.e_ins == 1) launch_bins<true>(k, wk, a.n, st, host_bin_counts);
   else launch_bins<false>(k, wk, a.n, st, host_bin_counts);
   hipLaunchKernelGGL(ext_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, (const int32_t*)wk.rec);

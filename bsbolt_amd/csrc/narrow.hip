@@ -1,0 +1,557 @@
+// Narrow-band ksw_extend2 with an exact acceptance certificate (learned-band form).
+//
+// The reference call runs at band W (the task's w clamped as ksw.c:399-407).
+// The narrow run N applies the same recurrences with the tighter cap WN < W on
+// both sides of the diagonal: beg >= i - WN, end <= i + WN + 1.  N's result is
+// returned only if it is provably identical, in all six outputs and the cell
+// count, to the W run; otherwise the task is flagged for the plain kernel.
+//
+// Certificate.  Claim: W's cells outside N's band are all zero, so W and N hold
+// the same eh[] state wherever N reads it and derive the same beg/end.  It holds
+// at row 0 and is kept, row by row, by three O(1) checks:
+//   (R1) when W's end lies beyond N's, the first column past N's end has no
+//        first-row insertion value left: max(h0 - oe_ins - (j-1)*e_ins, 0) == 0
+//        at j = N's end (ksw.c:395-397).  Columns past it were never written by
+//        N, so W reads only zeros there;
+//   (R2) no insertion value leaves N's band to the right: F at N's end is 0;
+//   (L)  no non-zero column is dropped on the left: the column that leaves N's
+//        band at the next row (j = i - WN) is not the first non-zero column of
+//        the row, which W's zero scan (ksw.c:466-467) would keep in its band.
+// Under the claim W's row maximum, z-drop and zero scans see N's values, and
+// W's band is beg_W = max(beg, i-W), end_W = min(end, i+W+1, qlen) with N's
+// beg/end.  What W computes beyond N is zeros: the cell count uses W's band,
+// and the end-to-end column (ksw.c:450-453) reads H(i, qlen-1), which is N's
+// value when qlen-1 is in N's band and 0 otherwise.
+//
+// Layout.  Diagonal frame: frame position k = g*C + c (lane g, column c) holds
+// column j = i - WN + k at row i, so a task costs WF = L*C >= 2*WN+2 positions
+// per row instead of its query length.  G (= H(i-1, j-1)) and E stay in place
+// during a row exactly as in the plain kernel; after the row the frame moves one
+// column right (G and E shift one position left) and the entering column starts
+// from its first-row value.  Positions left of column 0 and dead columns (left of
+// beg) hold G = E = 0, which makes their H zero and their F contribution <= 0, so
+// no mask against them is needed.  F keys are taken relative to the frame
+// ((k+1)*e_ins instead of (j+1)*e_ins): the offset is the same for every column
+// of a row and cancels.  The query profile is staged per task in LDS, indexed by
+// j + WN, so frame position k of row i reads entry i + k.
+#include "bsx_internal.h"
+#include "ext_common.h"
+
+namespace bsx {
+
+__host__ __device__ constexpr int band_of_class(int c) { return 8 << c; }  // 8, 16, 32, 64
+__host__ __device__ constexpr int frame_lanes(int c) { return c == 3 ? 32 : 16; }
+__host__ __device__ constexpr int frame_cols(int c) { return c == 0 ? 2 : c == 1 ? 3 : 5; }
+__host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) * frame_cols(c); }
+
+__device__ __forceinline__ int ramp_at(int j, int h0, int oe_ins, int e_ins) {  // eh[j].h before row 0
+  const int g = j == 0 ? h0 : h0 - oe_ins - (j - 1) * e_ins;
+  return g > 0 ? g : 0;
+}
+
+__device__ __forceinline__ uint32_t prof_word(const ExtKArgs& A, uint32_t qc) {
+  uint32_t lo = A.prof_lo[4];
+  lo = qc == 0 ? A.prof_lo[0] : lo;
+  lo = qc == 1 ? A.prof_lo[1] : lo;
+  lo = qc == 2 ? A.prof_lo[2] : lo;
+  lo = qc == 3 ? A.prof_lo[3] : lo;
+  return lo;
+}
+
+// every lane of the group receives the bitwise OR
+template <int L>
+__device__ __forceinline__ int grp_or(int x, int lane) {
+  if (L == 16) {
+    x |= __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true);
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true);
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true);
+    x |= __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true);
+    return x;
+  }
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+  if (L >= 64) x |= __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+  if (L == 64) return __builtin_amdgcn_readlane(x, 63);
+  const int a = __builtin_amdgcn_readlane(x, 31), b = __builtin_amdgcn_readlane(x, 63);
+  return lane < 32 ? a : b;
+}
+
+template <int L, int C, int WN, bool E1>
+__global__ void __launch_bounds__(256)
+ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
+                  int cls, uint8_t* __restrict__ need_plain) {
+  constexpr int WF = L * C;
+  static_assert(WF >= 2 * WN + 2, "the frame must hold the band and the column at its end");
+  constexpr int SZ = WN + kNarrowQMax + WF;  // profile entries per group
+  constexpr int NG = 256 / L;
+  constexpr int QD = (kNarrowQMax + 6) / 4;  // aligned dwords that can cover a query
+  constexpr int QR = (QD + L - 1) / L;       // query dword loads per lane
+  __shared__ uint32_t prof_lds[NG * SZ];
+  __shared__ uint32_t lds_next;
+  const int lane = threadIdx.x & 63;
+  const int lig = lane & (L - 1);
+  const int base = lig * C;
+  uint32_t* __restrict__ P = prof_lds + (threadIdx.x / L) * SZ;
+  uint32_t lbase = 0;
+  for (int b = 0; b < cls; ++b) lbase += counts[kExtBins + b];
+  const uint32_t nlist = counts[kExtBins + cls];
+  const uint32_t blk0 = blockIdx.x * (uint32_t)kExtTasksPerBlock;
+  if (blk0 >= nlist) return;  // uniform over the block
+  const uint32_t blk1 = min(blk0 + (uint32_t)kExtTasksPerBlock, nlist);
+  if (threadIdx.x == 0) lds_next = blk0;
+  __syncthreads();
+  const ExtDesc* __restrict__ desc = desc_all + lbase;
+  const int e_del = E1 ? 1 : A.e_del, e_ins = E1 ? 1 : A.e_ins;
+  const int oe_del = A.o_del + e_del, oe_ins = A.o_ins + e_ins;
+
+  auto next_pos = [&]() -> uint32_t {
+    uint32_t mine = 0;
+    if (lig == 0) mine = atomicAdd(&lds_next, 1u);
+    const uint32_t p = __shfl(mine, 0, L);
+    return p < blk1 ? p : blk1;
+  };
+  // ---- prefetch: descriptor two tasks ahead, sequence words of the next task ----
+  bool d2_ok = false;
+  int4 d2a = make_int4(0, 0, 0, 0), d2b = d2a, d2c = d2a;
+  auto load_desc = [&]() {
+    const uint32_t p = next_pos();
+    d2_ok = p < blk1;
+    const int4* d = (const int4*)(desc + (d2_ok ? p : blk0));
+    d2a = d[0]; d2b = d[1]; d2c = d[2];
+  };
+  bool nx_ok = false;
+  int nx_task = 0, nx_qlen = 0, nx_tlen = 0, nx_h0 = 0, nx_w = 0, nx_sh = 0;
+  uint64_t nx_qoff = 0, nx_toff = 0;
+  uint32_t nx_qd[QR];  // aligned query dwords lig + L*r (clamped to the query's last dword)
+  uint32_t nx_tb[4];   // target bytes [4*lig, 4*lig+4) (clamped)
+  auto stage1 = [&]() {
+    nx_ok = d2_ok;
+    nx_task = d2a.x; nx_qlen = d2a.y; nx_tlen = d2a.z; nx_h0 = d2a.w;
+    nx_w = d2b.x;
+    nx_qoff = (uint64_t)(uint32_t)d2b.z | ((uint64_t)(uint32_t)d2b.w << 32);
+    nx_toff = (uint64_t)(uint32_t)d2c.x | ((uint64_t)(uint32_t)d2c.y << 32);
+    // aligned dwords never cross a page, so the bytes read around the query are mapped
+    const uintptr_t qa = (uintptr_t)(A.query + nx_qoff);
+    const uint32_t* qp = (const uint32_t*)(qa & ~(uintptr_t)3);
+    nx_sh = (int)(qa & 3);
+    const int nd = (nx_sh + (nx_qlen > 0 ? nx_qlen : 1) + 3) >> 2;
+#pragma unroll
+    for (int r = 0; r < QR; ++r) nx_qd[r] = qp[min(lig + L * r, nd - 1)];
+    const uint8_t* tp = nx_tlen > 0 ? A.target + nx_toff : A.query + nx_qoff;
+    const int tmax = nx_tlen > 0 ? nx_tlen - 1 : 0;
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) nx_tb[k2] = tp[min(4 * lig + k2, tmax)];
+  };
+  auto load_window = [&](const uint8_t* tp, int tlen_, int first, uint32_t* out) {
+    const int tmax = tlen_ > 0 ? tlen_ - 1 : 0;
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) out[k2] = tp[min(first + 4 * lig + k2, tmax)];
+  };
+
+  int G[C], E[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) { G[c] = 0; E[c] = 0; }
+#pragma unroll
+  for (int r = 0; r < QR; ++r) nx_qd[r] = 0;
+  int task = 0, qlen = 0, tlen = 0, h0 = 0, W = 0, i = 0, ba = 0, ea = 0;
+  int best = 0, best_i = -1, best_j = -1, gbest = -1, gbest_i = -1, moff = 0;
+  uint32_t ncell = 0, tch = 0;
+  uint32_t tnx[4] = {0, 0, 0, 0};
+  uint64_t qoff = 0;
+  const uint8_t* tbase = A.target;
+  int tcb = 0;
+  bool need = true, done = false;
+
+  load_desc();
+  stage1();
+  load_desc();
+  for (;;) {
+    if (need && !done) {
+      if (!nx_ok) {
+        done = true;
+      } else {
+        task = nx_task; qlen = nx_qlen; tlen = nx_tlen; h0 = nx_h0; W = nx_w; qoff = nx_qoff;
+        // query profile of columns [-WN, qlen + WF - WN) at LDS entries [0, qlen + WF)
+#pragma unroll
+        for (int r = 0; r < QR; ++r) {
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int p = 4 * (lig + L * r) + b - nx_sh;
+            if (p >= 0 && p < qlen) P[p + WN] = prof_word(A, (nx_qd[r] >> (8 * b)) & 0xffu);
+          }
+        }
+        for (int x = lig; x < WN; x += L) P[x] = A.prof_lo[4];
+        for (int x = WN + qlen + lig; x < WN + qlen + WF; x += L) P[x] = A.prof_lo[4];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int c = 0; c < C; ++c) {  // row-0 frame: column j = k - WN (ksw.c:395-397)
+          const int j = base + c - WN;
+          G[c] = (j >= 0 && j <= qlen) ? ramp_at(j, h0, oe_ins, e_ins) : 0;
+          E[c] = 0;
+        }
+        {
+          uint32_t v = 0;
+#pragma unroll
+          for (int k2 = 0; k2 < 4; ++k2) v |= (4 * lig + k2 < tlen ? (nx_tb[k2] & 0xffu) : 0u) << (8 * k2);
+          tch = v;
+        }
+        tcb = 0;
+        i = 0; ba = 0; ea = qlen;
+        best = h0; best_i = best_j = -1; gbest = -1; gbest_i = -1; moff = 0;
+        ncell = 0;
+        tbase = tlen > 0 ? A.target + nx_toff : A.query + nx_qoff;
+        load_window(tbase, tlen, 4 * L, tnx);
+        stage1();
+        load_desc();
+        if (tlen <= 0) {
+          if (lig < 8) {
+            const int v = lig == 0 ? h0 : lig == 4 ? -1 : 0;
+            A.rec[(int64_t)task * 8 + lig] = v;
+          }
+        } else {
+          need = false;
+        }
+      }
+    }
+    if (__all(done)) break;
+    if (need || done) continue;
+
+    // ---------------- one row of the narrow run ----------------
+    if (i - tcb >= 4 * L) {
+      tcb = i;
+      uint32_t v = 0;
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v |= (i + 4 * lig + k2 < tlen ? (tnx[k2] & 0xffu) : 0u) << (8 * k2);
+      tch = v;
+      load_window(tbase, tlen, i + 4 * L, tnx);
+    }
+    const int tr = i - tcb;
+    const int t = (int)((__shfl(tch, tr >> 2, L) >> (8 * (tr & 3))) & 0xffu);
+    const bool tN = t > 3;
+
+    const int j0 = i - WN;  // column of frame position 0
+    const int bN = max(ba, j0), bW = max(ba, i - W);
+    const int eN = min(min(ea, i + WN + 1), qlen), eW = min(min(ea, i + W + 1), qlen);
+    const int kN = eN - j0;  // frame position of N's end
+    const int k0 = -j0;      // frame position of column 0
+    bool fail = eW > eN && ramp_at(eN, h0, oe_ins, e_ins) > 0;  // (R1)
+    int h1p = 0;
+    if (bN == 0) {
+      h1p = h0 - (A.o_del + e_del * (i + 1));
+      h1p = h1p > 0 ? h1p : 0;
+    }
+
+    int M[C], ex[C];
+    // rows past qlen + WN never run (their band is empty, ksw.c:460), so the clamp never binds
+    const uint32_t* __restrict__ Pr = P + min(i, WN + kNarrowQMax);
+#pragma unroll
+    for (int c = 0; c < C; ++c) M[c] = sbyte(Pr[base + c], t & 3);
+    if (__builtin_expect(__any(tN), 0)) {  // target base N: mat[20 + query code]
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int j = j0 + base + c;
+        const uint32_t qc = (j >= 0 && j < qlen) ? (uint32_t)A.query[qoff + j] : 4u;
+        int sN = A.prof_hi[4];
+        sN = qc == 0 ? A.prof_hi[0] : sN;
+        sN = qc == 1 ? A.prof_hi[1] : sN;
+        sN = qc == 2 ? A.prof_hi[2] : sN;
+        sN = qc == 3 ? A.prof_hi[3] : sN;
+        M[c] = tN ? sN : M[c];
+      }
+    }
+    int run = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int g = G[c];
+      const int mm = g ? g + M[c] : 0;
+      M[c] = mm;
+      const int key = max(mm - oe_ins, 0) + (base + c + 1) * e_ins;
+      ex[c] = run;
+      run = max(run, key);
+    }
+    const int lex = grp_excl_from_incl<L>(grp_incl_max<L>(run), lig);
+
+    int h[C];
+    int mk = 0, hf = 0, hik = 0, lok = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int k = base + c;
+      const int x = max(ex[c], lex);
+      const int f = x - k * e_ins;
+      const int hh = max(max(M[c], E[c]), f);
+      h[c] = hh;
+      const int ne = max(max(E[c] - e_del, M[c] - oe_del), 0);
+      if (k < kN) {
+        mk = max(mk, (hh << 11) | k);
+        E[c] = ne;
+      }
+      if (k == kN - 1) hf |= hh << 1;
+      if (k == kN && f > 0) hf |= 1;  // F(eN) > 0: an insertion leaves the band
+    }
+    const int hsh = __builtin_amdgcn_update_dpp(0, h[C - 1], L == 16 ? 0x111 : 0x138, 0xf, 0xf, true);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int k = base + c;
+      int left = c == 0 ? (lig == 0 ? 0 : hsh) : h[c - 1];
+      if (k == k0) left = h1p;
+      if (k <= kN) G[c] = left;
+      if (k == kN) E[c] = 0;
+      if ((k < kN) & ((G[c] | E[c]) != 0)) {
+        hik = max(hik, k + 1);
+        lok = max(lok, 2048 - k);
+      }
+    }
+    mk = grp_max<L>(mk, lane);
+    hf = grp_or<L>(hf, lane);
+    hik = grp_max<L>(hik, lane);
+    lok = grp_max<L>(lok, lane);
+    if (eW > eN && eN > bN && (hf & 1)) fail = true;  // (R2)
+
+    // W's view of the row: cell count and end-to-end column (ksw.c:449-453)
+    const int h1N = eN > bN ? (hf >> 1) : h1p;
+    int h1pW = 0;
+    if (bW == 0) {
+      h1pW = h0 - (A.o_del + e_del * (i + 1));
+      h1pW = h1pW > 0 ? h1pW : 0;
+    }
+    const int h1W = eW > bW ? ((eW == eN && eN > bN) ? h1N : 0) : h1pW;
+    const int jW = eW > bW ? eW : bW;
+    if (jW == qlen) {
+      if (!(gbest > h1W)) gbest_i = i;
+      gbest = gbest > h1W ? gbest : h1W;
+    }
+    ncell += eW > bW ? (uint32_t)(eW - bW) : 0u;
+    const int m = eN > bN ? mk >> 11 : 0;
+    const int mj = (mk & 2047) + j0;
+    bool fin = false;
+    if (!fail) {
+      if (m == 0) {
+        fin = true;
+      } else if (m > best) {
+        best = m; best_i = i; best_j = mj;
+        const int off = mj > i ? mj - i : i - mj;
+        moff = moff > off ? moff : off;
+      } else if (A.zdrop > 0) {
+        const int di = i - best_i, dj = mj - best_j;
+        const int drop = di > dj ? best - m - (di - dj) * e_del : best - m - (dj - di) * e_ins;
+        if (drop > A.zdrop) fin = true;
+      }
+      if (!fin) {  // ksw.c:466-469 on N's values
+        const int nb = lok > 0 ? 2048 - lok + j0 : eN;
+        const int hz = h1N != 0 ? eN : (hik > 0 ? hik - 1 + j0 : -1);
+        ba = nb;
+        const int ne = hz >= 0 ? hz + 2 : nb + 1;
+        ea = ne < qlen ? ne : qlen;
+        ++i;
+        if (i >= tlen) fin = true;
+        else if (nb == j0) fail = true;  // (L): column i-1-WN leaves N's band but is W's first non-zero
+        // move the frame one column right; the entering column has its first-row value
+        const int jin = i - WN + WF - 1;
+        const int gin = jin <= qlen ? ramp_at(jin, h0, oe_ins, e_ins) : 0;
+        const int gnx = __builtin_amdgcn_update_dpp(0, G[0], L == 16 ? 0x101 : 0x130, 0xf, 0xf, true);
+        const int enx = __builtin_amdgcn_update_dpp(0, E[0], L == 16 ? 0x101 : 0x130, 0xf, 0xf, true);
+#pragma unroll
+        for (int c = 0; c < C - 1; ++c) { G[c] = G[c + 1]; E[c] = E[c + 1]; }
+        G[C - 1] = lig == L - 1 ? gin : gnx;
+        E[C - 1] = lig == L - 1 ? 0 : enx;
+      }
+    }
+    if (fail) {
+      if (lig == 0) need_plain[task] = 1;
+      need = true;
+    } else if (fin) {
+      if (lig < 8) {  // ksw.c:473-478, one field per lane
+        int v = best;
+        v = lig == 1 ? best_j + 1 : v;
+        v = lig == 2 ? best_i + 1 : v;
+        v = lig == 3 ? gbest_i + 1 : v;
+        v = lig == 4 ? gbest : v;
+        v = lig == 5 ? moff : v;
+        v = lig == 6 ? (int)ncell : v;
+        v = lig == 7 ? 0 : v;
+        A.rec[(int64_t)task * 8 + lig] = v;
+      }
+      need = true;
+    }
+  }
+}
+
+// ---- routing of the learned-band form ----
+
+__device__ __forceinline__ int ext_bin_limit(int b) {
+  int lim = 0;
+#pragma unroll
+  for (int q = 0; q < kExtBins; ++q) lim = q == b ? bin_limit(q) : lim;
+  return lim;
+}
+
+// Narrow class of every valid task (0xff: plain only) and the class counts.  A
+// task goes narrow when its class is narrow, its query fits the narrow kernels,
+// its clamped band is wider than the class band and (unless any_gain) the class
+// frame is narrower than the plain kernel's columns for its query.
+__global__ void __launch_bounds__(256)
+ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
+  __shared__ uint32_t hist[kBandClasses];
+  if (threadIdx.x < kBandClasses) hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int b = W.bin_of[t];
+    if (b == 0xff) continue;
+    const int c = plan.mode == kBandFixed ? plan.fixed : (int)plan.cls[t];
+    int nb = 0xff;
+    if (c >= 0 && c < kBandClasses - 1 && A.qlen[t] <= kNarrowQMax &&
+        ext_clamped_w(A, t) > band_of_class(c) && (plan.any_gain || frame_of_class(c) < ext_bin_limit(b)))
+      nb = c;
+    W.nbin_of[t] = (uint8_t)nb;
+    W.need_plain[t] = nb == 0xff ? 1 : 0;
+    if (nb != 0xff) atomicAdd(&hist[nb], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBandClasses && hist[threadIdx.x])
+    atomicAdd(&W.counts[kExtBins + threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256)
+ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
+  __shared__ uint32_t off[kBandClasses], hist[kBandClasses], cur[kBandClasses];
+  const int64_t t0 = (int64_t)blockIdx.x * per_block;
+  const int64_t t1 = min(A.n, t0 + per_block);
+  if (threadIdx.x < kBandClasses) { hist[threadIdx.x] = 0; cur[threadIdx.x] = 0; }
+  __syncthreads();
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    if (W.bin_of[t] == 0xff) continue;
+    const int nb = W.nbin_of[t];
+    if (nb != 0xff) atomicAdd(&hist[nb], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBandClasses) {
+    uint32_t s = 0;
+    for (int b = 0; b < (int)threadIdx.x; ++b) s += W.counts[kExtBins + b];
+    const uint32_t hc = hist[threadIdx.x];
+    off[threadIdx.x] = s + (hc ? atomicAdd(&W.cursor[kExtBins + threadIdx.x], hc) : 0u);
+  }
+  __syncthreads();
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    if (W.bin_of[t] == 0xff) continue;
+    const int nb = W.nbin_of[t];
+    if (nb == 0xff) continue;
+    const uint32_t p = off[nb] + atomicAdd(&cur[nb], 1u);
+    write_desc(A, t, W.desc + p);
+  }
+}
+
+template <int CL, bool E1>
+static void launch_narrow_class(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st) {
+  constexpr int L = frame_lanes(CL), C = frame_cols(CL), WN = band_of_class(CL);
+  const int64_t blocks = (n + kExtTasksPerBlock - 1) / kExtTasksPerBlock;
+  if (blocks < 1) return;
+  hipLaunchKernelGGL((ext_narrow_kernel<L, C, WN, E1>), dim3((unsigned)blocks), dim3(256), 0, st, k,
+                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, CL, wk.need_plain);
+}
+
+template <bool E1>
+static void launch_narrow_classes(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st,
+                                  const BandPlan& plan) {
+  if (plan.mode == kBandFixed) {  // one class only
+    switch (plan.fixed) {
+      case 0: launch_narrow_class<0, E1>(k, wk, n, st); break;
+      case 1: launch_narrow_class<1, E1>(k, wk, n, st); break;
+      case 2: launch_narrow_class<2, E1>(k, wk, n, st); break;
+      case 3: launch_narrow_class<3, E1>(k, wk, n, st); break;
+      default: break;
+    }
+    return;
+  }
+  launch_narrow_class<3, E1>(k, wk, n, st);
+  launch_narrow_class<2, E1>(k, wk, n, st);
+  launch_narrow_class<1, E1>(k, wk, n, st);
+  launch_narrow_class<0, E1>(k, wk, n, st);
+}
+
+hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st,
+                         const BandPlan& plan) {
+  int64_t blocks = (a.n + 255) / 256;
+  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  hipLaunchKernelGGL(ext_narrow_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, plan);
+  const int64_t per_block = (a.n + blocks - 1) / blocks;
+  hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
+  if (a.e_del == 1 && a.e_ins == 1) launch_narrow_classes<true>(k, wk, a.n, st, plan);
+  else launch_narrow_classes<false>(k, wk, a.n, st, plan);
+  return hipGetLastError();
+}
+
+// ---- labels and statistics ----
+
+// After a fixed-class run of class c: a task whose narrow result was accepted,
+// or whose clamped band is not wider than the class band (the narrow run is
+// then the reference run), gets label c.  Classes are run from 3 down to 0, so
+// the label ends as the smallest accepted class; kBandClasses-1 = none.
+__global__ void __launch_bounds__(256)
+band_label_merge_kernel(ExtArgs A, ExtWork W, int c, uint8_t* __restrict__ labels, int first) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (W.bin_of[t] == 0xff) {
+      labels[t] = 0xff;
+      continue;
+    }
+    uint8_t lab = first ? (uint8_t)(kBandClasses - 1) : labels[t];
+    const bool trivial = A.qlen[t] <= kNarrowQMax && ext_clamped_w(A, t) <= band_of_class(c);
+    const bool ok = W.nbin_of[t] == c && W.need_plain[t] == 0;
+    if (trivial || ok) lab = (uint8_t)c;
+    labels[t] = lab;
+  }
+}
+
+// stats[2c] = tasks routed to class c (c = kBandClasses-1: plain only),
+// stats[2c+1] = of those, rejected by the certificate
+__global__ void __launch_bounds__(256)
+band_stats_kernel(int64_t n, const uint8_t* __restrict__ bin_of, const uint8_t* __restrict__ nbin_of,
+                  const uint8_t* __restrict__ need_plain, unsigned long long* __restrict__ stats) {
+  __shared__ uint32_t h[2 * kBandClasses];
+  if (threadIdx.x < 2 * kBandClasses) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    if (bin_of[t] == 0xff) continue;
+    const int c = nbin_of[t];
+    if (c == 0xff) {
+      atomicAdd(&h[2 * (kBandClasses - 1)], 1u);
+      continue;
+    }
+    atomicAdd(&h[2 * c], 1u);
+    if (need_plain[t]) atomicAdd(&h[2 * c + 1], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * kBandClasses && h[threadIdx.x])
+    atomicAdd(&stats[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ void band_stats_clear_kernel(unsigned long long* stats) {
+  if (threadIdx.x < 2 * kBandClasses) stats[threadIdx.x] = 0;
+}
+
+hipError_t launch_band_label_merge(const ExtArgs& a, const ExtWork& wk, int c, uint8_t* labels, int first,
+                                   int num_cu, hipStream_t st) {
+  int64_t blocks = (a.n + 255) / 256;
+  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  if (blocks < 1) return hipSuccess;
+  hipLaunchKernelGGL(band_label_merge_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, c, labels, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_band_stats(int64_t n, const ExtWork& wk, unsigned long long* stats, int num_cu, hipStream_t st) {
+  hipLaunchKernelGGL(band_stats_clear_kernel, dim3(1), dim3(64), 0, st, stats);
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  if (blocks < 1) return hipGetLastError();
+  hipLaunchKernelGGL(band_stats_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n, (const uint8_t*)wk.bin_of,
+                     (const uint8_t*)wk.nbin_of, (const uint8_t*)wk.need_plain, stats);
+  return hipGetLastError();
+}
+
+}  // namespace bsx

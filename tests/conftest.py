@@ -49,3 +49,21 @@ def engine():
     e = Engine(0)
     yield e
     e.close()
+
+
+def workload_tasks(cfg, scale):
+    """Extension batch of a scaled workload, its rounds run on the CPU oracle."""
+    from bsbolt_amd.workload import Workload, CONFIGS
+    from oracle import oracle as O
+    w = Workload(CONFIGS[cfg].scaled(scale))
+    try:
+        b, res, side, rnd = w.build_ext_batch(lambda t: O.extend2_batch(t, nthreads=8)[0])
+    finally:
+        w.close()
+    return b
+
+
+@pytest.fixture(scope="session")
+def wl_small():
+    """~5k cfg0-style tasks (100 bp SE) and ~5.5k cfg3-style tasks (250 bp, indels)."""
+    return workload_tasks("cfg0", 0.15), workload_tasks("cfg3", 0.004)
