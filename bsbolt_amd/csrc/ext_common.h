@@ -5,8 +5,17 @@
 namespace bsx {
 
 // ---- DPP group primitives (identity 0, values >= 0) ----
+// A group is L = 8, 16, 32 or 64 consecutive lanes of the wave.
+
 template <int L>
 __device__ __forceinline__ int grp_incl_max(int x) {
+  if (L == 8) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x90, 0xf, 0xf, false));  // quad_perm [0,0,1,2]
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x40, 0xf, 0xf, false));  // quad_perm [0,0,0,1]
+    const int q = __builtin_amdgcn_update_dpp(0, x, 0xFF, 0xf, 0xf, false);  // quad_perm [3,3,3,3]
+    // row_shr:4 into banks 1 and 3 only (lanes 4-7 of each 8-lane group)
+    return max(x, __builtin_amdgcn_update_dpp(0, q, 0x114, 0xf, 0xa, false));
+  }
   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true));  // row_shr:1
   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true));  // row_shr:2
   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true));  // row_shr:4
@@ -20,18 +29,19 @@ __device__ __forceinline__ int grp_incl_max(int x) {
 template <int L>
 __device__ __forceinline__ int grp_excl_from_incl(int incl, int lig) {
   if (L == 16) return __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
-  const int e = __builtin_amdgcn_update_dpp(0, incl, 0x138, 0xf, 0xf, true);        // wave_shr:1
+  const int e = L == 8 ? __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true)   // row_shr:1
+                       : __builtin_amdgcn_update_dpp(0, incl, 0x138, 0xf, 0xf, true);  // wave_shr:1
   return lig == 0 ? 0 : e;
 }
 
 // every lane of the group receives the maximum
 template <int L>
 __device__ __forceinline__ int grp_max(int x, int lane) {
-  if (L == 16) {
+  if (L <= 16) {
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true));   // quad_perm [1,0,3,2]
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true));   // quad_perm [2,3,0,1]
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true));  // row_half_mirror
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true));  // row_mirror
+    if (L == 16) x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true));  // row_mirror
     return x;
   }
   x = grp_incl_max<L>(x);
@@ -52,8 +62,10 @@ struct ExtKArgs {
   const uint8_t* target;
   int32_t* rec;  // [n][8]: score qle tle gtle gscore max_off cells_lo cells_hi
   int o_del, e_del, o_ins, e_ins, zdrop;
-  uint32_t prof_lo[5];
-  int32_t prof_hi[5];
+  uint32_t prof_lo[5];  // by query code q: bytes t = 0..3 hold mat[t*5+q]
+  int32_t prof_hi[5];   // by query code q: mat[20+q]
+  uint32_t trow_lo[5];  // by target code t: bytes q = 0..3 hold mat[t*5+q]
+  uint32_t trow_hi[5];  // by target code t: byte 0 holds mat[t*5+4]
 };
 
 // Band of a task clamped as ksw.c:399-407, in the same double arithmetic.

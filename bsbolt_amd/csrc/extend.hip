@@ -466,6 +466,15 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   k.query = a.query; k.target = a.target; k.rec = wk.rec;
   k.o_del = a.o_del; k.e_del = a.e_del; k.o_ins = a.o_ins; k.e_ins = a.e_ins; k.zdrop = a.zdrop;
   for (int q = 0; q < 5; ++q) { k.prof_lo[q] = a.prof_lo[q]; k.prof_hi[q] = a.prof_hi[q]; }
+  for (int t = 0; t < 5; ++t) {  // the same matrix by target code
+    uint32_t lo = 0;
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t b = t < 4 ? (a.prof_lo[q] >> (8 * t)) & 0xffu : (uint32_t)a.prof_hi[q] & 0xffu;
+      lo |= b << (8 * q);
+    }
+    k.trow_lo[t] = lo;
+    k.trow_hi[t] = t < 4 ? (a.prof_lo[4] >> (8 * t)) & 0xffu : (uint32_t)a.prof_hi[4] & 0xffu;
+  }
   hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, band ? 1 : 0);
   if (band) {
     hipError_t e = launch_narrow(a, k, wk, num_cu, st, *plan);

@@ -6,7 +6,7 @@ namespace bsx {
 
 constexpr int kBandFeatures = 24;  // features per task
 constexpr int kFeatPad = 32;       // stored features per task (BF16, zero padded)
-constexpr int kFeatScan = 64;      // query bases covered by the ungapped feature scans
+constexpr int kFeatScan = 16;      // query bases covered by the ungapped feature scans
 constexpr int kHidden = 64;
 constexpr int kOffW1 = 0;                                   // W1[64][24]
 constexpr int kOffB1 = kOffW1 + kHidden * kBandFeatures;    // b1[64]

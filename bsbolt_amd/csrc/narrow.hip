@@ -40,32 +40,35 @@
 namespace bsx {
 
 __host__ __device__ constexpr int band_of_class(int c) { return 8 << c; }  // 8, 16, 32, 64
-__host__ __device__ constexpr int frame_lanes(int c) { return c == 3 ? 32 : 16; }
-__host__ __device__ constexpr int frame_cols(int c) { return c == 0 ? 2 : c == 1 ? 3 : 5; }
+__host__ __device__ constexpr int frame_lanes(int c) { return c <= 1 ? 8 : 16; }
+__host__ __device__ constexpr int frame_cols(int c) { return c == 0 ? 3 : c == 1 ? 5 : c == 2 ? 5 : 9; }
 __host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) * frame_cols(c); }
+
+// Estimated VALU issue cost of one DP row per task (wave instructions / tasks
+// per wave): a fixed part per row plus a part per column held by each lane.
+// A task goes to a narrow class only where that class is cheaper than the
+// plain kernel's bin for its query length.
+__host__ __device__ constexpr int row_cost(int lanes, int cols, int fixed) {
+  return (fixed + 45 * cols) * lanes / 64;
+}
+__host__ __device__ constexpr bool narrow_pays(int c, int b) {  // >= 20% cheaper
+  return 10 * row_cost(frame_lanes(c), frame_cols(c), 140) <
+         8 * row_cost(kExtShape[b].lanes, kExtShape[b].cols, 110);
+}
 
 __device__ __forceinline__ int ramp_at(int j, int h0, int oe_ins, int e_ins) {  // eh[j].h before row 0
   const int g = j == 0 ? h0 : h0 - oe_ins - (j - 1) * e_ins;
   return g > 0 ? g : 0;
 }
 
-__device__ __forceinline__ uint32_t prof_word(const ExtKArgs& A, uint32_t qc) {
-  uint32_t lo = A.prof_lo[4];
-  lo = qc == 0 ? A.prof_lo[0] : lo;
-  lo = qc == 1 ? A.prof_lo[1] : lo;
-  lo = qc == 2 ? A.prof_lo[2] : lo;
-  lo = qc == 3 ? A.prof_lo[3] : lo;
-  return lo;
-}
-
 // every lane of the group receives the bitwise OR
 template <int L>
 __device__ __forceinline__ int grp_or(int x, int lane) {
-  if (L == 16) {
+  if (L <= 16) {
     x |= __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true);
     x |= __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true);
     x |= __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true);
-    x |= __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true);
+    if (L == 16) x |= __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true);
     return x;
   }
   x |= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
@@ -85,16 +88,21 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
                   int cls, uint8_t* __restrict__ need_plain) {
   constexpr int WF = L * C;
   static_assert(WF >= 2 * WN + 2, "the frame must hold the band and the column at its end");
-  constexpr int SZ = WN + kNarrowQMax + WF;  // profile entries per group
   constexpr int NG = 256 / L;
   constexpr int QD = (kNarrowQMax + 6) / 4;  // aligned dwords that can cover a query
   constexpr int QR = (QD + L - 1) / L;       // query dword loads per lane
-  __shared__ uint32_t prof_lds[NG * SZ];
+  // Query bytes of a group: byte WN + sh + j holds query[j] (sh = the query's
+  // offset in its first aligned dword); rows read bytes sh + i + k <= 3 + qlen + WN + WF - 1.
+  constexpr int SZW_STAGE = WN / 4 + L * QR;
+  constexpr int SZW_READ = (WN + WF + kNarrowQMax + 3) / 4 + 1;
+  constexpr int SZW = SZW_STAGE > SZW_READ ? SZW_STAGE : SZW_READ;  // dwords per group
+  __shared__ uint32_t q_lds[NG * SZW];
   __shared__ uint32_t lds_next;
   const int lane = threadIdx.x & 63;
   const int lig = lane & (L - 1);
   const int base = lig * C;
-  uint32_t* __restrict__ P = prof_lds + (threadIdx.x / L) * SZ;
+  uint32_t* __restrict__ Q32 = q_lds + (threadIdx.x / L) * SZW;
+  const uint8_t* __restrict__ Q8 = (const uint8_t*)Q32;
   uint32_t lbase = 0;
   for (int b = 0; b < cls; ++b) lbase += counts[kExtBins + b];
   const uint32_t nlist = counts[kExtBins + cls];
@@ -160,7 +168,7 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
   int best = 0, best_i = -1, best_j = -1, gbest = -1, gbest_i = -1, moff = 0;
   uint32_t ncell = 0, tch = 0;
   uint32_t tnx[4] = {0, 0, 0, 0};
-  uint64_t qoff = 0;
+  int sh = 0;
   const uint8_t* tbase = A.target;
   int tcb = 0;
   bool need = true, done = false;
@@ -173,18 +181,11 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
       if (!nx_ok) {
         done = true;
       } else {
-        task = nx_task; qlen = nx_qlen; tlen = nx_tlen; h0 = nx_h0; W = nx_w; qoff = nx_qoff;
-        // query profile of columns [-WN, qlen + WF - WN) at LDS entries [0, qlen + WF)
+        task = nx_task; qlen = nx_qlen; tlen = nx_tlen; h0 = nx_h0; W = nx_w;
+        // raw query dwords; bytes outside the query land only in columns that are never live
 #pragma unroll
-        for (int r = 0; r < QR; ++r) {
-#pragma unroll
-          for (int b = 0; b < 4; ++b) {
-            const int p = 4 * (lig + L * r) + b - nx_sh;
-            if (p >= 0 && p < qlen) P[p + WN] = prof_word(A, (nx_qd[r] >> (8 * b)) & 0xffu);
-          }
-        }
-        for (int x = lig; x < WN; x += L) P[x] = A.prof_lo[4];
-        for (int x = WN + qlen + lig; x < WN + qlen + WF; x += L) P[x] = A.prof_lo[4];
+        for (int r = 0; r < QR; ++r) Q32[WN / 4 + lig + L * r] = nx_qd[r];
+        sh = nx_sh;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -232,7 +233,13 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
     }
     const int tr = i - tcb;
     const int t = (int)((__shfl(tch, tr >> 2, L) >> (8 * (tr & 3))) & 0xffu);
-    const bool tN = t > 3;
+    // scores of this row's target base against query codes 0..4 (mat[t*5+q])
+    uint32_t s_lo = A.trow_lo[4], s_hi = A.trow_hi[4];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      s_lo = t == k2 ? A.trow_lo[k2] : s_lo;
+      s_hi = t == k2 ? A.trow_hi[k2] : s_hi;
+    }
 
     const int j0 = i - WN;  // column of frame position 0
     const int bN = max(ba, j0), bW = max(ba, i - W);
@@ -248,21 +255,11 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
 
     int M[C], ex[C];
     // rows past qlen + WN never run (their band is empty, ksw.c:460), so the clamp never binds
-    const uint32_t* __restrict__ Pr = P + min(i, WN + kNarrowQMax);
+    const uint8_t* __restrict__ Qr = Q8 + sh + min(i, WN + kNarrowQMax);
 #pragma unroll
-    for (int c = 0; c < C; ++c) M[c] = sbyte(Pr[base + c], t & 3);
-    if (__builtin_expect(__any(tN), 0)) {  // target base N: mat[20 + query code]
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const int j = j0 + base + c;
-        const uint32_t qc = (j >= 0 && j < qlen) ? (uint32_t)A.query[qoff + j] : 4u;
-        int sN = A.prof_hi[4];
-        sN = qc == 0 ? A.prof_hi[0] : sN;
-        sN = qc == 1 ? A.prof_hi[1] : sN;
-        sN = qc == 2 ? A.prof_hi[2] : sN;
-        sN = qc == 3 ? A.prof_hi[3] : sN;
-        M[c] = tN ? sN : M[c];
-      }
+    for (int c = 0; c < C; ++c) {
+      const uint32_t qc = Qr[base + c];  // query code of column j = i - WN + k
+      M[c] = (int)(int8_t)(__builtin_amdgcn_perm(s_hi, s_lo, qc) & 0xffu);
     }
     int run = 0;
 #pragma unroll
@@ -383,11 +380,13 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
 
 // ---- routing of the learned-band form ----
 
-__device__ __forceinline__ int ext_bin_limit(int b) {
-  int lim = 0;
+__device__ __forceinline__ bool narrow_pays_dev(int c, int b) {
+  bool r = false;
 #pragma unroll
-  for (int q = 0; q < kExtBins; ++q) lim = q == b ? bin_limit(q) : lim;
-  return lim;
+  for (int cc = 0; cc < kBandClasses - 1; ++cc)
+#pragma unroll
+    for (int q = 0; q < kExtBins; ++q) r = (cc == c && q == b) ? narrow_pays(cc, q) : r;
+  return r;
 }
 
 // Narrow class of every valid task (0xff: plain only) and the class counts.  A
@@ -406,7 +405,7 @@ ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
     const int c = plan.mode == kBandFixed ? plan.fixed : (int)plan.cls[t];
     int nb = 0xff;
     if (c >= 0 && c < kBandClasses - 1 && A.qlen[t] <= kNarrowQMax &&
-        ext_clamped_w(A, t) > band_of_class(c) && (plan.any_gain || frame_of_class(c) < ext_bin_limit(b)))
+        ext_clamped_w(A, t) > band_of_class(c) && (plan.any_gain || narrow_pays_dev(c, b)))
       nb = c;
     W.nbin_of[t] = (uint8_t)nb;
     W.need_plain[t] = nb == 0xff ? 1 : 0;

@@ -27,7 +27,7 @@ STRIDE = 32
 SHAPES = (("W1", (NH, NF)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)), ("W3", (NC, NH)), ("b3", (NC,)))
 NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 assert NPARAMS == 6085
-SCAN = 64
+SCAN = 16
 QMAX = 256
 WIDTHS = (8, 16, 32, 64)
 

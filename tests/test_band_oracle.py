@@ -123,21 +123,23 @@ def test_features_closed_form():
     qn[5] = 4
     b = ExtTasks.from_pairs([(q, t_same), (q, t_shift), (qn, t_same)], h0=[30, 30, 19], w=100, end_bonus=30)
     f = B.features_f32(b)
+    K = min(20, B.SCAN)  # bases covered by the ungapped scans
     # task 0: exact match of 20 bases
     assert f[0, 0] == 20 / 128 and f[0, 1] == 20 / 256 and f[0, 2] == 30 / 64
     assert f[0, 3] == min(100, 20 + 30 - 6 + 1) / 128  # clamped band (ksw.c:399-407)
     assert f[0, 4] == 30 / 32 and f[0, 5] == (30 - 7) / 64 and f[0, 6] == (30 - 7) / 64
     assert f[0, 7] == (30 - 7 - 20) / 64
-    assert f[0, 8] == 20 / 64 and f[0, 9] == 20 / 64 and f[0, 10] == 20 / 128 and f[0, 11] == 20 / 64
-    assert f[0, 12] == 20 / 128 and f[0, 13] == 16 / 16 and f[0, 18] == 0 and f[0, 19] == 0
-    assert f[0, 20] == (q == 3).sum() / 128 and f[0, 23] == 0
-    assert f[0, 21] == ((q == 1) | (q == 2)).sum() / 128
-    assert f[0, 22] == ((q[:-1] == 1) & (q[1:] == 2)).sum() / 32
+    assert f[0, 8] == K / 64 and f[0, 9] == K / 64 and f[0, 10] == K / 128 and f[0, 11] == K / 64
+    assert f[0, 12] == K / 128 and f[0, 13] == min(K, 16) / 16 and f[0, 18] == 0 and f[0, 19] == 0
+    qk = q[:K]
+    assert f[0, 20] == (qk == 3).sum() / 128 and f[0, 23] == 0
+    assert f[0, 21] == ((qk == 1) | (qk == 2)).sum() / 128
+    assert f[0, 22] == ((q[:K] == 1) & (q[1:K + 1] == 2)).sum() / 32
     # task 1: the +1 diagonal matches everywhere
-    assert f[1, 14] == 20 / 64
-    # task 2: one N in the query (mat[*][4] = -1)
-    assert f[2, 23] == 1 / 8 and f[2, 12] == 5 / 128 and f[2, 8] == 19 / 64
-    assert f[2, 11] == (19 - 1) / 64 and f[2, 9] == 18 / 64
+    assert f[1, 14] == K / 64
+    # task 2: one N in the query at position 5 (mat[*][4] = -1)
+    assert f[2, 23] == 1 / 8 and f[2, 12] == 5 / 128 and f[2, 8] == (K - 1) / 64
+    assert f[2, 11] == (K - 1 - 1) / 64 and f[2, 9] == (K - 2) / 64
 
 
 def test_bf16_rounding():

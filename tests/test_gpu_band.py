@@ -104,6 +104,26 @@ def test_fixed_class_random_scorings(engine, cls):
             assert np.array_equal(got[k], want[k]), (sc, k)
 
 
+@pytest.mark.parametrize("name", ["ext_random_bsbolt.npz", "ext_edge_bsbolt.npz", "ext_brute_bsbolt.npz",
+                                  "ext_random_nozdrop.npz", "ext_random_asym.npz", "ext_random_matrix.npz"])
+def test_fixed_class_on_reference_golden(engine, name):
+    """Narrow runs (w = 8 and 32) with their fallback reproduce the reference's
+    own outputs, including other scoring matrices and gap costs."""
+    from bsbolt_amd import bsx
+    from conftest import golden_ext
+    from test_gpu_extend import scoring_of
+    t, sc, ref = golden_ext(name)
+    for cls in (0, 2):
+        try:
+            engine.set_band_policy(bsx.BAND_FIXED, cls)
+            got = engine.extend(t, scoring_of(sc))
+        finally:
+            engine.set_band_policy(bsx.BAND_PLAIN)
+        for k in FIELDS:
+            bad = np.nonzero(got[k] != ref[k])[0]
+            assert len(bad) == 0, f"{name} class {cls}: {k} differs at {len(bad)} tasks, first {bad[:5]}"
+
+
 def _feats_labels(engine, b):
     return engine.band_features(b), engine.band_labels(b)
 
