@@ -8,19 +8,28 @@ namespace bsx {
 // Extension work is split into bins by query length; each bin runs a kernel
 // instantiation with L lanes per task and C query columns per lane
 // (qlen <= L*C).  Index = bin id.
-constexpr int kExtBins = 8;
+// Row cost per task ~ (fixed + per-column work * C) * L / 64 wave instructions,
+// so narrow groups (L = 16) with more columns beat wide groups up to C = 8.
+constexpr int kExtBins = 11;
 struct BinShape { int lanes, cols; };
 constexpr BinShape kExtShape[kExtBins] = {
-    {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
-constexpr int kExtSlices = 8;           // fetch-counter slots per bin (CIGAR path)
+    {16, 1}, {16, 2}, {16, 3}, {16, 4}, {16, 6}, {16, 8}, {32, 6}, {32, 8}, {64, 8}, {64, 16}, {64, 32}};
 constexpr int kExtTasksPerBlock = 128;  // task descriptors owned by one extension block
 constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
+
+// The CIGAR (ksw_global2) kernel bins tasks the same way with its own shapes.
+constexpr int kGlbBins = 8;
+constexpr BinShape kGlbShape[kGlbBins] = {
+    {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
+constexpr int kExtSlices = 8;  // fetch-counter slots per CIGAR bin
+constexpr int glb_bin_limit(int b) { return kGlbShape[b].lanes * kGlbShape[b].cols; }
 
 // Learned-band form: band classes 0..3 run the narrow kernel at w = 8, 16, 32,
 // 64; class 4 is the task's own band (plain kernel).  Count slots: plain bins
 // [0, kExtBins), narrow classes [kExtBins, kExtBins + kBandClasses - 1).
 constexpr int kBandClasses = 5;
 constexpr int kCountSlots = 16;
+static_assert(kExtBins + kBandClasses - 1 <= kCountSlots, "count slots for bins and narrow classes");
 constexpr int kNarrowQMax = 256;  // longest query the narrow kernels take
 enum BandMode { kBandPlain = 0, kBandLearned = 1, kBandFixed = 2 };
 struct BandPlan {

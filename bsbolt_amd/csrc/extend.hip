@@ -442,17 +442,16 @@ static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, hipS
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
 }
 
+template <int B, bool E1>
+static void launch_bins_from(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st, const int64_t* hc) {
+  launch_bin<B, E1>(k, wk, n, st, hc ? hc[B] : -1);
+  if constexpr (B > 0) launch_bins_from<B - 1, E1>(k, wk, n, st, hc);
+}
+
 template <bool E1>
 static void launch_bins(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st, const int64_t* hc) {
   // longest queries first so the long tasks do not form the tail
-  launch_bin<7, E1>(k, wk, n, st, hc ? hc[7] : -1);
-  launch_bin<6, E1>(k, wk, n, st, hc ? hc[6] : -1);
-  launch_bin<5, E1>(k, wk, n, st, hc ? hc[5] : -1);
-  launch_bin<4, E1>(k, wk, n, st, hc ? hc[4] : -1);
-  launch_bin<3, E1>(k, wk, n, st, hc ? hc[3] : -1);
-  launch_bin<2, E1>(k, wk, n, st, hc ? hc[2] : -1);
-  launch_bin<1, E1>(k, wk, n, st, hc ? hc[1] : -1);
-  launch_bin<0, E1>(k, wk, n, st, hc ? hc[0] : -1);
+  launch_bins_from<kExtBins - 1, E1>(k, wk, n, st, hc);
 }
 
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,

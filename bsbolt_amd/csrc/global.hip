@@ -232,15 +232,15 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
 
 __device__ __forceinline__ int glb_bin_of(int qlen) {
 #pragma unroll
-  for (int b = 0; b < kExtBins; ++b)
-    if (qlen <= bin_limit(b)) return b;
+  for (int b = 0; b < kGlbBins; ++b)
+    if (qlen <= glb_bin_limit(b)) return b;
   return -1;
 }
 
 __global__ void __launch_bounds__(256)
 glb_bin_count_kernel(GlbArgs A, GlbWork W) {
-  __shared__ uint32_t hist[kExtBins];
-  if (threadIdx.x < kExtBins) hist[threadIdx.x] = 0;
+  __shared__ uint32_t hist[kGlbBins];
+  if (threadIdx.x < kGlbBins) hist[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -260,15 +260,15 @@ glb_bin_count_kernel(GlbArgs A, GlbWork W) {
     W.bin_of[t] = (uint8_t)b;
   }
   __syncthreads();
-  if (threadIdx.x < kExtBins && hist[threadIdx.x]) atomicAdd(&W.counts[threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x < kGlbBins && hist[threadIdx.x]) atomicAdd(&W.counts[threadIdx.x], hist[threadIdx.x]);
 }
 
 __global__ void __launch_bounds__(256)
 glb_bin_scatter_kernel(int64_t n, GlbWork W) {
-  __shared__ uint32_t off[kExtBins];
+  __shared__ uint32_t off[kGlbBins];
   if (threadIdx.x == 0) {
     uint32_t s = 0;
-    for (int b = 0; b < kExtBins; ++b) { off[b] = s; s += W.counts[b]; }
+    for (int b = 0; b < kGlbBins; ++b) { off[b] = s; s += W.counts[b]; }
   }
   __syncthreads();
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -282,7 +282,7 @@ glb_bin_scatter_kernel(int64_t n, GlbWork W) {
 
 __global__ void glb_reset_kernel(GlbWork W) {
   const int k = threadIdx.x;
-  if (k < kExtBins) { W.counts[k] = 0; W.cursor[k] = 0; W.fetch[k] = 0; }
+  if (k < kGlbBins) { W.counts[k] = 0; W.cursor[k] = 0; W.fetch[k] = 0; }
   if (k == 0) { W.err[0] = 0; W.err[1] = 0x7fffffff; W.err[2] = 0; }
 }
 
@@ -294,7 +294,7 @@ int64_t glb_groups(int num_cu) {
 
 template <int B>
 static void glb_launch_bin(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st) {
-  constexpr int L = kExtShape[B].lanes, C = kExtShape[B].cols;
+  constexpr int L = kGlbShape[B].lanes, C = kGlbShape[B].cols;
   hipLaunchKernelGGL((glb_kernel<L, C>), dim3((unsigned)(num_cu * kGlbBlocksPerCU)), dim3(256), 0, st, a, w,
                      B);
 }
