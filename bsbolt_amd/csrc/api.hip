@@ -156,6 +156,10 @@ bsx_ctx* bsx_create(int device) {
   c->err_host[0] = 0;
   c->err_host[1] = INT_MAX;
   c->err_host[2] = 0;
+  // work granularity of the extension kernels (tuning knob; defaults in bsx_internal.h)
+  const char* tp = getenv("BSX_TPB_PLAIN");
+  const char* tn = getenv("BSX_TPB_NARROW");
+  set_tasks_per_block(tp ? atoi(tp) : kExtTasksPerBlock, tn ? atoi(tn) : kNarrowTasksPerBlock);
   return c;
 }
 

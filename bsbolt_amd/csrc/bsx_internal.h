@@ -14,7 +14,10 @@ constexpr int kExtBins = 11;
 struct BinShape { int lanes, cols; };
 constexpr BinShape kExtShape[kExtBins] = {
     {16, 1}, {16, 2}, {16, 3}, {16, 4}, {16, 6}, {16, 8}, {32, 6}, {32, 8}, {64, 8}, {64, 16}, {64, 32}};
-constexpr int kExtTasksPerBlock = 128;  // task descriptors owned by one extension block
+constexpr int kExtBlock = 64;             // threads per extension block: one wave, so a block
+                                          // never holds wave slots idle while a sibling finishes
+constexpr int kExtTasksPerBlock = 32;     // task descriptors owned by one plain-kernel block
+constexpr int kNarrowTasksPerBlock = 16;  // ... by one narrow-kernel block
 constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
 
 // The CIGAR (ksw_global2) kernel bins tasks the same way with its own shapes.
@@ -88,6 +91,9 @@ struct ExtWork {
 struct ExtKArgs;
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts, const BandPlan* plan);
+// Tasks per block of the plain and narrow kernels (defaults above; a context
+// may override them, e.g. from BSX_TPB_PLAIN / BSX_TPB_NARROW).
+void set_tasks_per_block(int plain, int narrow);
 hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st,
                          const BandPlan& plan);
 

@@ -66,6 +66,7 @@ struct ExtKArgs {
   int32_t prof_hi[5];   // by query code q: mat[20+q]
   uint32_t trow_lo[5];  // by target code t: bytes q = 0..3 hold mat[t*5+q]
   uint32_t trow_hi[5];  // by target code t: byte 0 holds mat[t*5+4]
+  int tpb;              // task descriptors owned by one block
 };
 
 // Band of a task clamped as ksw.c:399-407, in the same double arithmetic.

@@ -43,7 +43,7 @@ namespace bsx {
 
 
 template <int L, int C, bool E1>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kExtBlock)
 ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
                  int bin) {
   __shared__ uint32_t lds_next;
@@ -53,9 +53,9 @@ ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_
   uint32_t lbase = 0;
   for (int b = 0; b < bin; ++b) lbase += counts[b];
   const uint32_t nlist = counts[bin];
-  const uint32_t blk0 = blockIdx.x * (uint32_t)kExtTasksPerBlock;
+  const uint32_t blk0 = blockIdx.x * (uint32_t)A.tpb;
   if (blk0 >= nlist) return;  // uniform over the block
-  const uint32_t blk1 = min(blk0 + (uint32_t)kExtTasksPerBlock, nlist);
+  const uint32_t blk1 = min(blk0 + (uint32_t)A.tpb, nlist);
   if (threadIdx.x == 0) lds_next = blk0;
   __syncthreads();
   const ExtDesc* __restrict__ desc = desc_all + lbase;
@@ -433,12 +433,12 @@ ext_unpack_kernel(ExtArgs A, ExtWork W, const int32_t* __restrict__ rec) {
 template <int B, bool E1>
 static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, hipStream_t st, int64_t hint) {
   constexpr int L = kExtShape[B].lanes, C = kExtShape[B].cols;
-  // one block per kExtTasksPerBlock descriptors of the bin; without a host-side
+  // one block per k.tpb descriptors of the bin; without a host-side
   // count the grid covers the whole batch and surplus blocks exit at once
   const int64_t n = hint >= 0 ? hint : n_all;
-  const int64_t blocks = (n + kExtTasksPerBlock - 1) / kExtTasksPerBlock;
+  const int64_t blocks = (n + k.tpb - 1) / k.tpb;
   if (blocks < 1) return;
-  hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(256), 0, st, k,
+  hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
 }
 
@@ -454,6 +454,13 @@ static void launch_bins(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStre
   launch_bins_from<kExtBins - 1, E1>(k, wk, n, st, hc);
 }
 
+static int g_tpb_plain = kExtTasksPerBlock, g_tpb_narrow = kNarrowTasksPerBlock;
+
+void set_tasks_per_block(int plain, int narrow) {
+  if (plain >= 16 && plain <= 65536) g_tpb_plain = plain;
+  if (narrow >= 16 && narrow <= 65536) g_tpb_narrow = narrow;
+}
+
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts, const BandPlan* plan) {
   hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(kExtBins * kExtSlices), 0, st, wk);
@@ -464,6 +471,7 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   ExtKArgs k{};
   k.query = a.query; k.target = a.target; k.rec = wk.rec;
   k.o_del = a.o_del; k.e_del = a.e_del; k.o_ins = a.o_ins; k.e_ins = a.e_ins; k.zdrop = a.zdrop;
+  k.tpb = g_tpb_plain;
   for (int q = 0; q < 5; ++q) { k.prof_lo[q] = a.prof_lo[q]; k.prof_hi[q] = a.prof_hi[q]; }
   for (int t = 0; t < 5; ++t) {  // the same matrix by target code
     uint32_t lo = 0;
@@ -476,7 +484,9 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   }
   hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, band ? 1 : 0);
   if (band) {
-    hipError_t e = launch_narrow(a, k, wk, num_cu, st, *plan);
+    ExtKArgs kn = k;
+    kn.tpb = g_tpb_narrow;
+    hipError_t e = launch_narrow(a, kn, wk, num_cu, st, *plan);
     if (e != hipSuccess || plan->skip_plain) return e;
     hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, 2);
     host_bin_counts = nullptr;

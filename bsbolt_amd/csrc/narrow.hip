@@ -83,12 +83,12 @@ __device__ __forceinline__ int grp_or(int x, int lane) {
 }
 
 template <int L, int C, int WN, bool E1>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kExtBlock)
 ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
                   int cls, uint8_t* __restrict__ need_plain) {
   constexpr int WF = L * C;
   static_assert(WF >= 2 * WN + 2, "the frame must hold the band and the column at its end");
-  constexpr int NG = 256 / L;
+  constexpr int NG = kExtBlock / L;
   constexpr int QD = (kNarrowQMax + 6) / 4;  // aligned dwords that can cover a query
   constexpr int QR = (QD + L - 1) / L;       // query dword loads per lane
   // Query bytes of a group: byte WN + sh + j holds query[j] (sh = the query's
@@ -106,9 +106,9 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
   uint32_t lbase = 0;
   for (int b = 0; b < cls; ++b) lbase += counts[kExtBins + b];
   const uint32_t nlist = counts[kExtBins + cls];
-  const uint32_t blk0 = blockIdx.x * (uint32_t)kExtTasksPerBlock;
+  const uint32_t blk0 = blockIdx.x * (uint32_t)A.tpb;
   if (blk0 >= nlist) return;  // uniform over the block
-  const uint32_t blk1 = min(blk0 + (uint32_t)kExtTasksPerBlock, nlist);
+  const uint32_t blk1 = min(blk0 + (uint32_t)A.tpb, nlist);
   if (threadIdx.x == 0) lds_next = blk0;
   __syncthreads();
   const ExtDesc* __restrict__ desc = desc_all + lbase;
@@ -448,9 +448,9 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
 template <int CL, bool E1>
 static void launch_narrow_class(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st) {
   constexpr int L = frame_lanes(CL), C = frame_cols(CL), WN = band_of_class(CL);
-  const int64_t blocks = (n + kExtTasksPerBlock - 1) / kExtTasksPerBlock;
+  const int64_t blocks = (n + k.tpb - 1) / k.tpb;
   if (blocks < 1) return;
-  hipLaunchKernelGGL((ext_narrow_kernel<L, C, WN, E1>), dim3((unsigned)blocks), dim3(256), 0, st, k,
+  hipLaunchKernelGGL((ext_narrow_kernel<L, C, WN, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, CL, wk.need_plain);
 }
 
