@@ -36,6 +36,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a copy of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"traffic_bytes_per_launch\s+(.+?)\s*=\s*(\d+)", line.strip())
+        if m and m.group(1) == kernel:
+            return int(m.group(2))
+    return None
+
+
 def time_steps(eng, ptrs, steps):
     eng.sync()
     eng.timer_start()
@@ -67,12 +81,15 @@ def learned_leg(args, eng, ptrs, outs, cells_d, plain_out, plain_cells, n, total
         eng.set_band_model(p)
         eng.set_band_policy(bsx.BAND_LEARNED)
         try:
-            for _ in range(args.warmup):
+            for _ in range(max(1, args.warmup)):
                 eng.extend_device(ptrs)
             eng.sync()
             mism = int(sum(int((outs[k].numpy() != plain_out[k]).sum()) for k in RESULT_FIELDS))
             mism += int((cells_d.numpy() != plain_cells).sum())
+            eng.set_kernel_timing(args.steps)
             ms = time_steps(eng, ptrs, args.steps)
+            kms, kcalls = eng.kernel_times()
+            eng.set_kernel_timing(0)
             st = eng.band_stats()
         finally:
             eng.set_band_policy(bsx.BAND_PLAIN)
@@ -83,7 +100,10 @@ def learned_leg(args, eng, ptrs, outs, cells_d, plain_out, plain_cells, n, total
                  "gcups": total_cells / (ms / 1e3) / 1e9, "mismatches_vs_plain": mism,
                  "routed": routed, "rejected": rejected,
                  "narrow_accepted": narrow - int(st[:4, 1].sum()),
-                 "fallback_rate": (int(st[:4, 1].sum()) / narrow) if narrow else 0.0}
+                 "fallback_rate": (int(st[:4, 1].sum()) / narrow) if narrow else 0.0,
+                 "kernel_ms_per_step": {"predict": float(kms[15] / max(kcalls, 1)),
+                                        "narrow": float(kms[11:15].sum() / max(kcalls, 1)),
+                                        "plain": float(kms[:11].sum() / max(kcalls, 1))}}
         log(f"learned ({name}): {ms:.3f} ms/step, routed {routed}, rejected {rejected}, mismatches {mism}")
         if name == "trained":
             res.update(entry)
@@ -104,7 +124,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-band leg")
     ap.add_argument("--band-model", default=None, help="band model (.npy); default bsbolt_amd/band_model.npy")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        from bsbolt_amd import bsx
+        import numpy as _np
+        eng = bsx.Engine(0)
+        a = eng.empty((args.copy_bytes,), _np.uint8)
+        b = eng.empty((args.copy_bytes,), _np.uint8)
+        eng._check(eng.L.bsx_memcpy(eng.ctx, b.ptr, a.ptr, args.copy_bytes, 3))
+        eng.timer_start()
+        eng._check(eng.L.bsx_memcpy(eng.ctx, b.ptr, a.ptr, args.copy_bytes, 3))
+        ms = eng.timer_stop()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "ms": ms,
+                          "GB/s": 2 * args.copy_bytes / (ms / 1e3) / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the extension batch shards "
                                    "by task index with no exchange (see DESIGN.md), multi-GPU runs are "
@@ -136,7 +171,7 @@ def main():
     ptrs.update({k: v.ptr for k, v in outs.items()})
     ptrs["cells"] = cells_d.ptr
     ptrs["n"] = n
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):  # at least one run: its outputs are verified below
         eng.extend_device(ptrs)
     eng.sync()
     got = {k: v.numpy() for k, v in outs.items()}
@@ -150,6 +185,7 @@ def main():
     # timed region: K steps bracketed by stream synchronisation; HIP events on
     # the stream the kernels run on
     eng.sync()
+    eng.set_kernel_timing(args.steps)  # HIP events around every kernel of the timed steps
     w0 = time.perf_counter()
     eng.timer_start()
     for _ in range(args.steps):
@@ -157,6 +193,8 @@ def main():
     ms = eng.timer_stop()
     eng.sync()
     wall = time.perf_counter() - w0
+    kms, kcalls = eng.kernel_times()
+    eng.set_kernel_timing(0)
     ms_step = ms / args.steps
     ext_s = n / (ms_step / 1e3)
     gcups = total_cells / (ms_step / 1e3) / 1e9
@@ -164,6 +202,20 @@ def main():
     alg_bytes = int(batch.qlen.astype(np.int64).sum() + batch.tlen.astype(np.int64).sum()) + n * (36 + 24 + 8)
     gbs = alg_bytes / (ms_step / 1e3) / 1e9
     log(f"{args.steps} steps: {ms_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} ms/step")
+    # dominant kernel: the plain bin with the most time; its algorithmic work is
+    # the cells of the tasks in that bin
+    limits = bsx.ext_bin_limits()
+    bin_of = np.searchsorted(limits, batch.qlen)
+    per_launch = kms[:len(limits)] / max(kcalls, 1)
+    dom = int(np.argmax(per_launch))
+    dom_name = f"ext_plain_kernel<{bsx.EXT_SHAPES[dom][0]}, {bsx.EXT_SHAPES[dom][1]}, true>"
+    dom_cells = int(cells[bin_of == dom].sum())
+    dom_ms = float(per_launch[dom])
+    dom_tops = dom_cells * OPS_PER_CELL / (dom_ms / 1e3) / 1e12
+    m = bin_of == dom
+    dom_bytes = int(batch.qlen[m].astype(np.int64).sum() + batch.tlen[m].astype(np.int64).sum()) + int(m.sum()) * 80
+    traffic = profiled_traffic(dom_name, args.workload)
+    log(f"dominant kernel {dom_name}: {dom_ms:.3f} ms/launch, {dom_tops:.2f} TOP/s, traffic {traffic}")
 
     # ---- learned-band form: same batch, same outputs required ----
     learned = None
@@ -195,9 +247,14 @@ def main():
         "plain": {"value": ext_s, "ms_per_step": ms_step, "gcups": gcups},
         "gcups": total_cells / (best_ms / 1e3) / 1e9,
         "gcups_nominal": nominal / (best_ms / 1e3) / 1e9,
-        "roofline": {"bound": "valu", "achieved": tops, "peak": PEAK_PK16_TOPS, "unit": "TOP/s",
-                     "frac": tops / PEAK_PK16_TOPS, "frac_int32": tops / PEAK_INT32_TOPS, "traffic": None,
-                     "ops_per_cell": OPS_PER_CELL, "kernel": "plain extension kernels"},
+        "roofline": {"bound": "valu", "kernel": dom_name, "achieved": dom_tops, "peak": PEAK_PK16_TOPS,
+                     "unit": "TOP/s", "frac": dom_tops / PEAK_PK16_TOPS, "frac_int32": dom_tops / PEAK_INT32_TOPS,
+                     "traffic": traffic, "ops_per_launch": dom_cells * OPS_PER_CELL, "ops_per_cell": OPS_PER_CELL,
+                     "ms_per_launch": dom_ms, "share_of_step": dom_ms / ms_step,
+                     "algorithmic_bytes_per_launch": dom_bytes,
+                     "kernel_ms_per_step": {f"bin<={int(l)}": float(t) for l, t in zip(limits, per_launch) if t > 0}},
+        "roofline_step": {"bound": "valu", "achieved": tops, "peak": PEAK_PK16_TOPS, "unit": "TOP/s",
+                          "frac": tops / PEAK_PK16_TOPS, "frac_int32": tops / PEAK_INT32_TOPS},
         "roofline_hbm": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes": alg_bytes},
         "learned_band": learned,

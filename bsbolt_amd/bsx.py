@@ -33,7 +33,10 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_dev_free", "bsx_memcpy", "bsx_timer_start", "bsx_timer_stop", "bsx_set_band_policy",
            "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get", "bsx_band_features_device",
            "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
-           "bsx_band_train_device")
+           "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits")
+TIMED_SLOTS = 16
+# lanes per task and columns per lane of the plain kernel's bins (bsx_internal.h kExtShape)
+EXT_SHAPES = ((16, 1), (16, 2), (16, 3), (16, 4), (16, 6), (16, 8), (32, 6), (32, 8), (64, 8), (64, 16), (64, 32))
 
 # learned-band form (include/bsx.h)
 BAND_PLAIN, BAND_LEARNED, BAND_FIXED = 0, 1, 2
@@ -130,6 +133,11 @@ def lib():
         L.bsx_band_forward_device.argtypes = [P, P, C.c_int64, P, P]
         L.bsx_band_grad_device.argtypes = [P, P, P, C.c_int64, P]
         L.bsx_band_train_device.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int, C.c_float, P]
+        L.bsx_set_kernel_timing.argtypes = [P, C.c_int]
+        L.bsx_kernel_times.argtypes = [P, P, C.c_int, P]
+        L.bsx_ext_bin_limits.argtypes = [P, C.c_int]
+        for f in ("bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits"):
+            getattr(L, f).restype = C.c_int
         for f in ("bsx_set_band_policy", "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get",
                   "bsx_band_features_device", "bsx_band_labels_device", "bsx_band_forward_device",
                   "bsx_band_grad_device", "bsx_band_train_device"):
@@ -138,6 +146,14 @@ def lib():
         L.bsx_ksw_extend2.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P] + [C.c_int] * 8 + [P] * 5
         _lib = L
     return _lib
+
+
+def ext_bin_limits():
+    """Largest query length of each plain-kernel bin."""
+    n = lib().bsx_ext_bin_limits(None, 0)
+    out = np.zeros(n, np.int32)
+    lib().bsx_ext_bin_limits(out.ctypes.data, n)
+    return out
 
 
 def fill_scmat(a=1, b=4):
@@ -243,6 +259,17 @@ class Engine:
                                     "end_bonus")])
         rs = _ExtResults(*[dev.get(k) for k in RESULT_FIELDS + ("cells",)])
         self._check(self.L.bsx_extend_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
+
+    # ---- per-kernel timing ----
+    def set_kernel_timing(self, max_calls):
+        self._check(self.L.bsx_set_kernel_timing(self.ctx, int(max_calls)))
+
+    def kernel_times(self):
+        """(ms summed per slot, number of calls recorded); slots: plain bins, narrow classes, predictor."""
+        ms = np.zeros(TIMED_SLOTS, np.float32)
+        calls = C.c_int(0)
+        self._check(self.L.bsx_kernel_times(self.ctx, ms.ctypes.data, TIMED_SLOTS, C.byref(calls)))
+        return ms, calls.value
 
     # ---- learned-band form ----
     def set_band_policy(self, policy, band_class=0):

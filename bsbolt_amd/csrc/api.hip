@@ -66,6 +66,10 @@ struct bsx_ctx {
   bool model_ready = false;
   int adam_step = 0;
   int64_t last_n = 0;  // tasks of the last extension call (for bsx_band_stats)
+  // per-kernel timing of the extension calls (bsx_set_kernel_timing)
+  std::vector<hipEvent_t> kt_ev;   // [calls][2 * kTimedSlots]
+  std::vector<uint32_t> kt_mask;   // [calls] slots recorded
+  int kt_calls = 0, kt_used = 0;
   DevBuf zbuf;                 // direction-byte slots of the global kernel
   int64_t zslot = 1 << 16;     // bytes per slot: largest min(qlen,2w+1)*tlen accepted
   // staging for the host-memory entry points
@@ -171,6 +175,7 @@ void bsx_destroy(bsx_ctx* c) {
                     &c->h_i32, &c->h_res, &c->nbin_of, &c->need_plain, &c->cls, &c->stats, &c->model, &c->part,
                     &c->grad})
     b->release();
+  for (hipEvent_t e : c->kt_ev) (void)hipEventDestroy(e);
   if (c->err_host) (void)hipHostFree(c->err_host);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -272,11 +277,19 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   rc = ext_buffers(c, n);
   if (rc) return rc;
   ExtWork wk = ext_work(c);
+  if (c->kt_used < c->kt_calls) {  // per-kernel timing of this call
+    wk.tev = &c->kt_ev[(size_t)c->kt_used * 2 * kTimedSlots];
+    wk.tmask = &c->kt_mask[c->kt_used];
+    *wk.tmask = 0;
+    ++c->kt_used;
+  }
   BandPlan plan{kBandPlain, 0, nullptr, false, false};
   if (c->band_policy == BSX_BAND_LEARNED) {
     if (!c->model_ready) return fail(c, BSX_EINVAL, "learned band policy without a band model (bsx_band_model_set)");
     HIPCHK(c, c->cls.ensure(n + 1), "hipMalloc(cls)");
+    time_begin(wk, kTimedSlots - 1, c->stream);
     HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), (uint8_t*)c->cls.p, c->num_cu, c->stream), "band predict");
+    time_end(wk, kTimedSlots - 1, c->stream);
     plan = BandPlan{kBandLearned, 0, (const uint8_t*)c->cls.p, false, false};
   } else if (c->band_policy == BSX_BAND_FIXED) {
     plan = BandPlan{kBandFixed, c->band_class, nullptr, true, false};
@@ -288,6 +301,46 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   c->pending_check = true;
   c->pending_kind = 1;
   return BSX_OK;
+}
+
+int bsx_set_kernel_timing(bsx_ctx* c, int max_calls) {
+  if (!c || max_calls < 0 || max_calls > 4096) return fail(c, BSX_EINVAL, "max_calls must be in [0, 4096]");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  const size_t need = (size_t)max_calls * 2 * kTimedSlots;
+  while (c->kt_ev.size() < need) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e), "hipEventCreate");
+    c->kt_ev.push_back(e);
+  }
+  c->kt_mask.assign((size_t)max_calls, 0u);
+  c->kt_calls = max_calls;
+  c->kt_used = 0;
+  return BSX_OK;
+}
+
+int bsx_kernel_times(bsx_ctx* c, float* ms, int n_slots, int* calls) {
+  if (!c || !ms || n_slots < 1) return fail(c, BSX_EINVAL, "bad argument");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (int s = 0; s < n_slots; ++s) ms[s] = 0.f;
+  for (int k = 0; k < c->kt_used; ++k)
+    for (int s = 0; s < kTimedSlots && s < n_slots; ++s) {
+      if (!(c->kt_mask[k] >> s & 1u)) continue;
+      float t = 0.f;
+      const hipEvent_t* e = &c->kt_ev[(size_t)k * 2 * kTimedSlots + 2 * s];
+      HIPCHK(c, hipEventElapsedTime(&t, e[0], e[1]), "hipEventElapsedTime");
+      ms[s] += t;
+    }
+  if (calls) *calls = c->kt_used;
+  c->kt_used = 0;
+  return BSX_OK;
+}
+
+int bsx_ext_bin_limits(int32_t* limits, int n) {
+  if (!limits) return kExtBins;
+  for (int b = 0; b < kExtBins && b < n; ++b) limits[b] = bin_limit(b);
+  return kExtBins;
 }
 
 int bsx_set_band_policy(bsx_ctx* c, int policy, int band_class) {

@@ -86,7 +86,23 @@ struct ExtWork {
   int32_t* err;        // [0] rejected count, [1] lowest rejected index
   uint8_t* nbin_of;    // [n] narrow class of the task, 0xff = plain only (learned-band form)
   uint8_t* need_plain; // [n] 1 = the plain kernel computes the task (learned-band form)
+  // optional per-kernel timing: events [2*slot], [2*slot+1] around each kernel
+  // launch (slots: plain bins, then narrow classes, then the predictor); *tmask
+  // gets bit `slot` set for every pair recorded
+  hipEvent_t* tev = nullptr;
+  uint32_t* tmask = nullptr;
 };
+constexpr int kTimedSlots = 16;  // kExtBins + 4 narrow classes + predictor
+static_assert(kExtBins + kBandClasses <= kTimedSlots, "timing slots");
+inline void time_begin(const ExtWork& wk, int slot, hipStream_t st) {
+  if (wk.tev) (void)hipEventRecord(wk.tev[2 * slot], st);
+}
+inline void time_end(const ExtWork& wk, int slot, hipStream_t st) {
+  if (wk.tev) {
+    (void)hipEventRecord(wk.tev[2 * slot + 1], st);
+    *wk.tmask |= 1u << slot;
+  }
+}
 
 struct ExtKArgs;
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,

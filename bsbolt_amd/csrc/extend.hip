@@ -438,8 +438,10 @@ static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, hipS
   const int64_t n = hint >= 0 ? hint : n_all;
   const int64_t blocks = (n + k.tpb - 1) / k.tpb;
   if (blocks < 1) return;
+  time_begin(wk, B, st);
   hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
+  time_end(wk, B, st);
 }
 
 template <int B, bool E1>

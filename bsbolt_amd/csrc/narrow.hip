@@ -450,8 +450,10 @@ static void launch_narrow_class(const ExtKArgs& k, const ExtWork& wk, int64_t n,
   constexpr int L = frame_lanes(CL), C = frame_cols(CL), WN = band_of_class(CL);
   const int64_t blocks = (n + k.tpb - 1) / k.tpb;
   if (blocks < 1) return;
+  time_begin(wk, kExtBins + CL, st);
   hipLaunchKernelGGL((ext_narrow_kernel<L, C, WN, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, CL, wk.need_plain);
+  time_end(wk, kExtBins + CL, st);
 }
 
 template <bool E1>

@@ -192,6 +192,18 @@ int bsx_band_grad_device(bsx_ctx *ctx, const uint16_t *feats, const uint8_t *lab
 int bsx_band_train_device(bsx_ctx *ctx, const uint16_t *feats, const uint8_t *labels, int64_t n,
                           int64_t batch, int steps, float lr, float *losses);
 
+/* ---- per-kernel timing (measurement; no counterpart in the reference) ----
+ * bsx_set_kernel_timing(ctx, k): the next k extension calls on the context
+ * record HIP events around each kernel launch (0 switches it off).
+ * bsx_kernel_times: waits, then writes the summed milliseconds per slot over
+ * the recorded calls (slots 0..bins-1 = plain kernels by query-length bin,
+ * then the 4 narrow classes, slot 15 = band predictor), the number of calls
+ * in *calls, and starts a new recording.  bsx_ext_bin_limits writes the
+ * largest query length of each plain bin and returns the number of bins. */
+int bsx_set_kernel_timing(bsx_ctx *ctx, int max_calls);
+int bsx_kernel_times(bsx_ctx *ctx, float *ms, int n_slots, int *calls);
+int bsx_ext_bin_limits(int32_t *limits, int n);
+
 /* Scalar drop-in with the exact signature of ksw_extend2 (ksw.h:108), run as
  * a one-task batch on the calling thread's default context (device 0).  m must
  * be 5.  On a precondition violation or device error it returns INT32_MIN and

@@ -37,3 +37,9 @@ def test_library_has_no_oracle_dependency():
     # the product library must not link or embed the CPU oracle
     data = open(os.path.join(ROOT, "bsbolt_amd", "libbsx.so"), "rb").read()
     assert b"orc_extend2" not in data and b"liboracle" not in data
+
+
+def test_bin_shapes_match_library():
+    from bsbolt_amd import bsx
+    lim = bsx.ext_bin_limits()
+    assert list(lim) == [l * c for l, c in bsx.EXT_SHAPES]
