@@ -215,11 +215,21 @@ ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_
       h1p = h1p > 0 ? h1p : 0;
     }
 
+    // live columns of this lane: c < d (j < end).  Every other band mask of
+    // the row derives from these C+1 compares (scalar mask logic):
+    //   j <= end  <=>  c-1 < d;   j == end-1  <=>  lt[c] && !lt[c+1];   j == end  <=>  le && !lt[c]
+    const int d = end - base;
+    bool lt[C + 1];
+#pragma unroll
+    for (int c = 0; c <= C; ++c) lt[c] = c < d;
+    const bool le0 = d >= 0;
+
     int M[C], ex[C];
     // substitution scores of this row; a target base N (code 4, which the
     // 2-bit reference never produces) takes the slow path, decided per wave
+    const uint32_t sh8 = 8u * (uint32_t)(t & 3);
 #pragma unroll
-    for (int c = 0; c < C; ++c) M[c] = sbyte(plo[c], t & 3);
+    for (int c = 0; c < C; ++c) M[c] = __builtin_amdgcn_sbfe((int)plo[c], sh8, 8u);
     if (__builtin_expect(__any(tN), 0)) {
 #pragma unroll
       for (int c = 0; c < C; ++c) {
@@ -235,47 +245,49 @@ ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_
     int run = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int s = M[c];
       const int g = G[c];
-      const int mm = g ? g + s : 0;
+      const int mm = g ? g + M[c] : 0;
       M[c] = mm;
-      // key + e_ins = max(M - oe_ins, 0) + (j+1)*e_ins  (>= 1)
-      const int key = max(mm - oe_ins, 0) + kE0 + (c + 1) * e_ins;
+      // key + e_ins = max(M - oe_ins, 0) + (j+1)*e_ins = max(M + (j+1)*e_ins - oe_ins, (j+1)*e_ins)  (>= 1)
+      const int kb = kE0 + (c + 1) * e_ins;
+      const int key = max(mm + (kb - oe_ins), kb);
       ex[c] = run;
       run = max(run, key);
     }
     const int lex = grp_excl_from_incl<L>(grp_incl_max<L>(run), lig);
 
     int h[C];
-    int mk = 0, h1c = 0, hik = 0, lok = 0;
+    int mk = 0, h1c = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int j = base + c;
-      const bool live = j < end;
       const int x = max(ex[c], lex);
       const int f = x - (kE0 + c * e_ins);  // F(j); 0 when no column precedes j
       const int hh = max(max(M[c], E[c]), f);
       h[c] = hh;
       const int ne = max(max(E[c] - e_del, M[c] - oe_del), 0);
-      if (live) {
-        mk = max(mk, (hh << 11) | j);
-        E[c] = ne;
-      }
-      if (j == end - 1) h1c = hh;
+      mk = lt[c] ? max(mk, (hh << 11) | j) : mk;
+      E[c] = lt[c] ? ne : E[c];
+      h1c = (lt[c] && !lt[c + 1]) ? hh : h1c;  // j == end - 1
     }
     // G(j) = H(i, j-1) for j <= end; column 0 receives the first-column value
     const int hsh = __builtin_amdgcn_update_dpp(0, h[C - 1], L == 16 ? 0x111 : 0x138, 0xf, 0xf, true);
     const int hprev = lig == 0 ? h1p : hsh;
+    uint32_t nzb = 0;  // bit c: column c holds a non-zero G or E after the row
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int j = base + c;
       const int left = c == 0 ? hprev : h[c - 1];
-      if (j <= end) G[c] = left;
-      if (j == end) E[c] = 0;
-      if ((j < end) & ((G[c] | E[c]) != 0)) {
-        hik = max(hik, j + 1);
-        lok = max(lok, 2048 - j);
-      }
+      const bool le = c == 0 ? le0 : lt[c - 1];
+      G[c] = le ? left : G[c];
+      E[c] = (le && !lt[c]) ? 0 : E[c];  // j == end
+      nzb |= (uint32_t)min(G[c] | E[c], 1) << c;
+    }
+    // only columns j < end count: keep bits c < d
+    nzb &= d >= C ? 0xffffffffu : d > 0 ? (1u << d) - 1u : 0u;
+    int hik = 0, lok = 0;  // last non-zero column + 1; 2048 - first non-zero column
+    if (nzb) {
+      hik = base + 32 - __builtin_clz(nzb);
+      lok = 2048 - base - __builtin_ctz(nzb);
     }
     mk = grp_max<L>(mk, lane);
     h1c = grp_max<L>(h1c, lane);
