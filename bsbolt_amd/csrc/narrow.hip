@@ -261,20 +261,27 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
       const uint32_t qc = Qr[base + c];  // query code of column j = i - WN + k
       M[c] = (int)(int8_t)(__builtin_amdgcn_perm(s_hi, s_lo, qc) & 0xffu);
     }
+    // frame positions k < kN are live; the other band masks derive from these compares
+    const int dN = kN - base;
+    bool lt[C + 1];
+#pragma unroll
+    for (int c = 0; c <= C; ++c) lt[c] = c < dN;
+    const bool le0 = dN >= 0;
     int run = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int g = G[c];
       const int mm = g ? g + M[c] : 0;
       M[c] = mm;
-      const int key = max(mm - oe_ins, 0) + (base + c + 1) * e_ins;
+      const int kb = (base + c + 1) * e_ins;
+      const int key = max(mm + (kb - oe_ins), kb);
       ex[c] = run;
       run = max(run, key);
     }
     const int lex = grp_excl_from_incl<L>(grp_incl_max<L>(run), lig);
 
     int h[C];
-    int mk = 0, hf = 0, hik = 0, lok = 0;
+    int mk = 0, hf = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int k = base + c;
@@ -283,25 +290,29 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
       const int hh = max(max(M[c], E[c]), f);
       h[c] = hh;
       const int ne = max(max(E[c] - e_del, M[c] - oe_del), 0);
-      if (k < kN) {
-        mk = max(mk, (hh << 11) | k);
-        E[c] = ne;
-      }
-      if (k == kN - 1) hf |= hh << 1;
-      if (k == kN && f > 0) hf |= 1;  // F(eN) > 0: an insertion leaves the band
+      mk = lt[c] ? max(mk, (hh << 11) | k) : mk;
+      E[c] = lt[c] ? ne : E[c];
+      hf = (lt[c] && !lt[c + 1]) ? (hf | (hh << 1)) : hf;  // k == kN - 1
+      const bool le = c == 0 ? le0 : lt[c - 1];
+      hf = (le && !lt[c] && f > 0) ? (hf | 1) : hf;  // k == kN: F(eN) > 0, an insertion leaves the band
     }
     const int hsh = __builtin_amdgcn_update_dpp(0, h[C - 1], L == 16 ? 0x111 : 0x138, 0xf, 0xf, true);
+    uint32_t nzb = 0;  // bit c: position c holds a non-zero G or E after the row
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int k = base + c;
       int left = c == 0 ? (lig == 0 ? 0 : hsh) : h[c - 1];
-      if (k == k0) left = h1p;
-      if (k <= kN) G[c] = left;
-      if (k == kN) E[c] = 0;
-      if ((k < kN) & ((G[c] | E[c]) != 0)) {
-        hik = max(hik, k + 1);
-        lok = max(lok, 2048 - k);
-      }
+      left = k == k0 ? h1p : left;
+      const bool le = c == 0 ? le0 : lt[c - 1];
+      G[c] = le ? left : G[c];
+      E[c] = (le && !lt[c]) ? 0 : E[c];
+      nzb |= (uint32_t)min(G[c] | E[c], 1) << c;
+    }
+    nzb &= dN >= C ? 0xffffffffu : dN > 0 ? (1u << dN) - 1u : 0u;
+    int hik = 0, lok = 0;
+    if (nzb) {
+      hik = base + 32 - __builtin_clz(nzb);
+      lok = 2048 - base - __builtin_ctz(nzb);
     }
     mk = grp_max<L>(mk, lane);
     hf = grp_or<L>(hf, lane);
