@@ -224,6 +224,148 @@ class Workload:
             (np.concatenate(rounds) if rounds else np.zeros(0, np.int32))
 
 
+def infer_bw(l1, l2, score, a, q, r):
+    """Band that can hold the gaps of a local hit of `score` (bwamem.c:796-803),
+    vectorised; C's (int) cast of the double quotient truncates toward zero."""
+    l1, l2, score = (np.asarray(x, np.int64) for x in (l1, l2, score))
+    w = np.trunc((np.minimum(l1, l2) * a - score - q) / r + 2.0).astype(np.int64)
+    w = np.maximum(w, np.abs(l1 - l2))
+    return np.where((l1 == l2) & (l1 * a - score < ((q + r - a) << 1)), 0, w)
+
+
+def cigar_band(w_, l_query, rlen, mat0=1, o_del=6, e_del=1, o_ins=6, e_ins=1):
+    """Global band of bwa_gen_cigar2 (bwa.c:231-241), vectorised."""
+    l_query, rlen, w_ = (np.asarray(x, np.int64) for x in (l_query, rlen, w_))
+    half = (l_query + 1) >> 1
+    max_ins = np.trunc((half * mat0 - o_ins) / e_ins + 1.0).astype(np.int64)
+    max_del = np.trunc((half * mat0 - o_del) / e_del + 1.0).astype(np.int64)
+    max_gap = np.maximum(np.maximum(max_ins, max_del), 1)
+    d = np.abs(rlen - l_query)
+    w = (max_gap + d + 1) >> 1
+    w = np.minimum(w, w_)
+    return np.maximum(w, d + 3)
+
+
+def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only=True):
+    """CIGAR pass over every alignment region of a workload, in rounds, as
+    mem_reg2aln (bwamem.c:1197-1233) and bwa_gen_cigar2 (bwa.c:199-248) run it:
+    band w2 = max of the inferred deletion/insertion bands (capped by the
+    region's band when above -w), then up to 3 tries doubling w2 (at most 4w)
+    while the global score stays below truesc - a and changes.  Each try of a
+    region reverses query and reference when rb >= l_pac (leftmost indels) and
+    runs ksw_global2 at the band bwa_gen_cigar2 derives from w2; a region with
+    w2 == 0 and equal lengths is scored ungapped without DP (bwa.c:222-230).
+
+    `global_fn(GlbTasks) -> (score, n_cigar, cigar[n, slot])` runs one round.
+    Returns (all issued tasks as one GlbTasks, their scores, n_cigar, CIGAR
+    lists, region index per task, round per task, final score per region)."""
+    o = dict(DEFAULT_OPT) if opt is None else opt
+    mat = fill_matrix(o["a"], o["b"]) if mat is None else np.asarray(mat, np.int8).reshape(25)
+    regs = []
+    for i in range(wl.n_reads):
+        rs_ = wl.regions(i)
+        if primary_only and len(rs_):  # the SAM record of a read comes from its best region
+            rs_ = rs_[np.argmax(rs_[:, 4]):][:1]
+        for r in rs_:
+            qb, qe, rb, re, score, truesc, w = (int(x) for x in r[:7])
+            if score < min_score or qe <= qb or rb >= re:
+                continue
+            regs.append((i, qb, qe, rb, re, truesc, w))
+    nreg = len(regs)
+    R = np.array([r[1:] for r in regs], np.int64).reshape(-1, 6)
+    qlen = R[:, 1] - R[:, 0]
+    rlen = R[:, 3] - R[:, 2]
+    truesc = R[:, 4]
+    tmp = infer_bw(qlen, rlen, truesc, o["a"], o["o_del"], o["e_del"])
+    w2 = np.maximum(infer_bw(qlen, rlen, truesc, o["a"], o["o_ins"], o["e_ins"]), tmp)
+    w2 = np.where(w2 > o["w"], np.minimum(w2, R[:, 5]), w2)
+    last = np.full(nreg, -(1 << 30), np.int64)
+    tries = np.zeros(nreg, np.int64)
+    final = np.full(nreg, -(1 << 30), np.int64)
+    active = np.ones(nreg, bool)
+    seqs = {}
+    parts, scores, ncig, cigs, owner, rnd = [], [], [], [], [], []
+    wmax = o["w"] << 2
+    for rr in range(3):
+        idx = np.nonzero(active)[0]
+        if len(idx) == 0:
+            break
+        w2[idx] = np.minimum(w2[idx], wmax)
+        pairs, pw, pidx, sc_round = [], [], [], np.zeros(len(idx), np.int64)
+        for k, g in enumerate(idx):
+            i, qb, qe, rb, re = regs[g][:5]
+            if g not in seqs:
+                qs = wl.read(i)[qb:qe].copy()
+                rs = wl.get_seq(rb, re)
+                if len(rs) != re - rb or (rb < wl.l_pac < re):
+                    seqs[g] = None
+                else:
+                    if rb >= wl.l_pac:  # leftmost indels on the reverse strand (bwa.c:216-221)
+                        qs, rs = qs[::-1].copy(), rs[::-1].copy()
+                    seqs[g] = (qs, rs)
+            if seqs[g] is None:
+                sc_round[k] = -(1 << 30)
+                continue
+            qs, rs = seqs[g]
+            if len(qs) == len(rs) and w2[g] == 0:  # ungapped (bwa.c:222-230)
+                sc_round[k] = int(mat[rs.astype(np.int64) * 5 + qs].sum())
+                continue
+            pairs.append((qs, rs))
+            pw.append(int(cigar_band(w2[g], len(qs), len(rs), mat[0], o["o_del"], o["e_del"], o["o_ins"],
+                                     o["e_ins"])))
+            pidx.append(k)
+        if pairs:
+            t = GlbTasks.from_pairs(pairs, pw)
+            s_, nc, cg = global_fn(t)
+            for j, k in enumerate(pidx):
+                sc_round[k] = int(s_[j])
+            parts.append(t)
+            scores.append(np.asarray(s_))
+            ncig.append(np.asarray(nc))
+            cigs.extend([list(cg[j, :nc[j]]) if nc[j] > 0 else [] for j in range(len(pairs))])
+            owner.append(idx[np.asarray(pidx, np.int64)])
+            rnd.append(np.full(len(pairs), rr, np.int32))
+        # do-while tail of mem_reg2aln (bwamem.c:1229-1233)
+        final[idx] = sc_round
+        stop = (sc_round == last[idx]) | (w2[idx] == wmax)
+        cont = ~stop
+        last[idx] = np.where(cont, sc_round, last[idx])
+        w2[idx] = np.where(cont, w2[idx] << 1, w2[idx])
+        tries[idx] += 1
+        cont &= (tries[idx] < 3) & (sc_round < truesc[idx] - o["a"])
+        active[:] = False
+        active[idx[cont]] = True
+    batch = concat_glb(parts)
+    cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
+    return (batch, cat(scores, np.int32), cat(ncig, np.int32), cigs, cat(owner, np.int64), cat(rnd, np.int32),
+            final)
+
+
+def fill_matrix(a=1, b=4):
+    """bwa_fill_scmat (bwa.c:169-178)."""
+    m = np.full((5, 5), -b, np.int8)
+    np.fill_diagonal(m, a)
+    m[4, :] = -1
+    m[:, 4] = -1
+    return m.reshape(25)
+
+
+def concat_glb(parts):
+    if not parts:
+        z = np.zeros(0, np.int32)
+        return GlbTasks(np.zeros(0, np.uint8), np.zeros(0, np.uint64), z, np.zeros(0, np.uint8),
+                        np.zeros(0, np.uint64), z, z)
+    qbase = np.cumsum([0] + [len(p.query) for p in parts[:-1]], dtype=np.uint64)
+    tbase = np.cumsum([0] + [len(p.target) for p in parts[:-1]], dtype=np.uint64)
+    return GlbTasks(np.concatenate([p.query for p in parts]),
+                    np.concatenate([p.q_off + qb for p, qb in zip(parts, qbase)]),
+                    np.concatenate([p.qlen for p in parts]),
+                    np.concatenate([p.target for p in parts]),
+                    np.concatenate([p.t_off + tb for p, tb in zip(parts, tbase)]),
+                    np.concatenate([p.tlen for p in parts]),
+                    np.concatenate([p.w for p in parts]))
+
+
 def concat_ext(parts):
     if not parts:
         z = np.zeros(0, np.int32)
