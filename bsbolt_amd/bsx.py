@@ -33,7 +33,8 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_dev_free", "bsx_memcpy", "bsx_timer_start", "bsx_timer_stop", "bsx_set_band_policy",
            "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get", "bsx_band_features_device",
            "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
-           "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits")
+           "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
+           "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain kernel's bins (bsx_internal.h kExtShape)
 EXT_SHAPES = ((16, 1), (16, 2), (16, 3), (16, 4), (16, 6), (16, 8), (32, 6), (32, 8), (64, 8), (64, 16), (64, 32))
@@ -133,6 +134,16 @@ def lib():
         L.bsx_band_forward_device.argtypes = [P, P, C.c_int64, P, P]
         L.bsx_band_grad_device.argtypes = [P, P, P, C.c_int64, P]
         L.bsx_band_train_device.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int, C.c_float, P]
+        L.bsx_graph_capture_extend.argtypes = [P, C.POINTER(Scoring), C.POINTER(_ExtTasks), C.POINTER(_ExtResults),
+                                               C.POINTER(C.c_void_p)]
+        L.bsx_graph_capture_global.argtypes = [P, C.POINTER(Scoring), C.POINTER(_GlbTasks), C.POINTER(_GlbResults),
+                                               C.POINTER(C.c_void_p)]
+        L.bsx_graph_capture_extend.restype = C.c_int
+        L.bsx_graph_capture_global.restype = C.c_int
+        L.bsx_graph_launch.argtypes = [P, P]
+        L.bsx_graph_launch.restype = C.c_int
+        L.bsx_graph_destroy.argtypes = [P]
+        L.bsx_graph_destroy.restype = None
         L.bsx_set_kernel_timing.argtypes = [P, C.c_int]
         L.bsx_kernel_times.argtypes = [P, P, C.c_int, P]
         L.bsx_ext_bin_limits.argtypes = [P, C.c_int]
@@ -260,6 +271,71 @@ class Engine:
         rs = _ExtResults(*[dev.get(k) for k in RESULT_FIELDS + ("cells",)])
         self._check(self.L.bsx_extend_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
 
+    # ---- graph-captured submission ----
+    def capture_extend(self, dev, scoring: Scoring = None):
+        """Graph of one device-memory extension batch (pointers as extend_device)."""
+        sc = scoring or Scoring.make()
+        tk = self._ext_tasks_dev(dev)
+        rs = _ExtResults(*[dev.get(k) for k in RESULT_FIELDS + ("cells",)])
+        h = C.c_void_p()
+        self._check(self.L.bsx_graph_capture_extend(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs), C.byref(h)))
+        return Graph(self, h.value)
+
+    def capture_global(self, dev, scoring: Scoring = None):
+        sc = scoring or Scoring.make()
+        tk = _GlbTasks(dev["n"], *[dev.get(k) for k in
+                                   ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")])
+        rs = _GlbResults(dev.get("score"), dev.get("n_cigar"), dev.get("cigar"), dev.get("cigar_slot", 0))
+        h = C.c_void_p()
+        self._check(self.L.bsx_graph_capture_global(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs), C.byref(h)))
+        return Graph(self, h.value)
+
+    def stream_global(self, tasks: GlbTasks, batch=262144, slot=None, scoring: Scoring = None):
+        """CIGARs of a host batch streamed through one captured graph per
+        `batch` tasks (the last, shorter chunk runs eagerly).  Returns
+        (score, n_cigar, cigar[n, slot]) like global_."""
+        sc = scoring or Scoring.make()
+        n = len(tasks)
+        if slot is None:
+            slot = int((tasks.qlen + tasks.tlen).max()) if n else 1
+        score = np.zeros(n, np.int32)
+        ncig = np.zeros(n, np.int32)
+        cig = np.zeros((n, slot), np.uint32)
+        full = (n // batch) * batch
+        if full:
+            from .tasks import GlbTasks as _G
+            chunks = [slice(a, a + batch) for a in range(0, full, batch)]
+            subs = []
+            for sl in chunks:
+                q0, q1 = int(tasks.q_off[sl.start]), int(tasks.q_off[sl.stop - 1]) + int(tasks.qlen[sl.stop - 1])
+                t0, t1 = int(tasks.t_off[sl.start]), int(tasks.t_off[sl.stop - 1]) + int(tasks.tlen[sl.stop - 1])
+                subs.append(_G(tasks.query[q0:q1], tasks.q_off[sl] - q0, tasks.qlen[sl], tasks.target[t0:t1],
+                               tasks.t_off[sl] - t0, tasks.tlen[sl], tasks.w[sl]))
+            fields = ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")
+            dev = {k: self.empty((max(getattr(b, k).nbytes for b in subs),), np.uint8) for k in fields}
+            ds, dn = self.empty((batch,), np.int32), self.empty((batch,), np.int32)
+            dc = self.empty((batch * slot,), np.uint32)
+            ptrs = {k: v.ptr for k, v in dev.items()}
+            ptrs.update(n=batch, score=ds.ptr, n_cigar=dn.ptr, cigar=dc.ptr, cigar_slot=slot)
+            for k in fields:
+                dev[k].upload(getattr(subs[0], k))
+            g = self.capture_global(ptrs, sc)
+            try:
+                for sl, b in zip(chunks, subs):
+                    for k in fields:
+                        dev[k].upload(getattr(b, k))
+                    g.launch()
+                    self.sync()
+                    score[sl], ncig[sl] = ds.numpy(), dn.numpy()
+                    cig[sl] = dc.numpy().reshape(batch, slot)
+            finally:
+                g.close()
+        if full < n:
+            rest = _glb_subset(tasks, full, n)
+            s_, nc_, cg_ = self.global_(rest, sc, cigar=True, slot=slot)
+            score[full:], ncig[full:], cig[full:] = s_, nc_, cg_
+        return score, ncig, cig
+
     # ---- per-kernel timing ----
     def set_kernel_timing(self, max_calls):
         self._check(self.L.bsx_set_kernel_timing(self.ctx, int(max_calls)))
@@ -378,6 +454,35 @@ class Engine:
         self._check(self.L.bsx_global_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
 
 
+def _glb_subset(t, a, b):
+    q0, t0 = int(t.q_off[a]), int(t.t_off[a])
+    q1 = int(t.q_off[b - 1]) + int(t.qlen[b - 1])
+    t1 = int(t.t_off[b - 1]) + int(t.tlen[b - 1])
+    return GlbTasks(t.query[q0:q1], t.q_off[a:b] - q0, t.qlen[a:b], t.target[t0:t1], t.t_off[a:b] - t0,
+                    t.tlen[a:b], t.w[a:b])
+
+
+class Graph:
+    """A captured batch call; launch() replays it on the context's stream."""
+
+    def __init__(self, eng, handle):
+        self.eng, self.h = eng, handle
+
+    def launch(self):
+        self.eng._check(self.eng.L.bsx_graph_launch(self.eng.ctx, self.h))
+
+    def close(self):
+        if self.h:
+            self.eng.L.bsx_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DevArray:
     """Device memory owned through a libbsx context (no other HIP runtime is
     needed in the process)."""
@@ -387,6 +492,14 @@ class DevArray:
         self.ptr = eng.L.bsx_dev_alloc(eng.ctx, max(1, nbytes))
         if not self.ptr:
             raise BsxError(BSX_ENOMEM, eng.last_error())
+
+    def upload(self, arr):
+        """Copies a host array into the start of this buffer (synchronous)."""
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        if arr.nbytes:
+            self.eng._check(self.eng.L.bsx_memcpy(self.eng.ctx, self.ptr, arr.ctypes.data, arr.nbytes, 1))
 
     def numpy(self):
         out = np.empty(self.shape, self.dtype)

@@ -51,16 +51,28 @@ struct DevBuf {
 
 }  // namespace
 
+// Device workspace of the batch entry points.  A context has its own; a
+// captured graph owns another, so later calls on the context can grow theirs
+// without moving memory the graph refers to.
+struct Work {
+  DevBuf bin_of, list, desc, rec, small;  // small: counts/cursor/err/fetch
+  DevBuf nbin_of, need_plain, cls;        // learned-band form
+  DevBuf zbuf;                            // direction-byte slots of the global kernel
+  void release() {
+    for (DevBuf* b : {&bin_of, &list, &desc, &rec, &small, &nbin_of, &need_plain, &cls, &zbuf}) b->release();
+  }
+};
+
 struct bsx_ctx {
   int device = 0;
   int num_cu = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   std::string err;
-  // extension workspace
-  DevBuf bin_of, list, desc, rec, small;  // small: counts/cursor/err/fetch
+  Work wown;           // workspace of the eager calls
+  Work* ws = &wown;    // workspace in use (a graph's while it is captured)
   // learned-band form
-  DevBuf nbin_of, need_plain, cls, stats, model, part, grad;
+  DevBuf stats, model, part, grad;
   int band_policy = BSX_BAND_PLAIN;
   int band_class = 0;
   bool model_ready = false;
@@ -70,7 +82,6 @@ struct bsx_ctx {
   std::vector<hipEvent_t> kt_ev;   // [calls][2 * kTimedSlots]
   std::vector<uint32_t> kt_mask;   // [calls] slots recorded
   int kt_calls = 0, kt_used = 0;
-  DevBuf zbuf;                 // direction-byte slots of the global kernel
   int64_t zslot = 1 << 16;     // bytes per slot: largest min(qlen,2w+1)*tlen accepted
   // staging for the host-memory entry points
   DevBuf h_query, h_target, h_qoff, h_toff, h_i32, h_res;
@@ -171,9 +182,9 @@ void bsx_destroy(bsx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->bin_of, &c->list, &c->desc, &c->rec, &c->small, &c->zbuf, &c->h_query, &c->h_target, &c->h_qoff, &c->h_toff,
-                    &c->h_i32, &c->h_res, &c->nbin_of, &c->need_plain, &c->cls, &c->stats, &c->model, &c->part,
-                    &c->grad})
+  c->wown.release();
+  for (DevBuf* b : {&c->h_query, &c->h_target, &c->h_qoff, &c->h_toff, &c->h_i32, &c->h_res, &c->stats, &c->model,
+                    &c->part, &c->grad})
     b->release();
   for (hipEvent_t e : c->kt_ev) (void)hipEventDestroy(e);
   if (c->err_host) (void)hipHostFree(c->err_host);
@@ -223,15 +234,15 @@ static MlpDev model_view(bsx_ctx* c) {
 
 static ExtWork ext_work(bsx_ctx* c) {
   ExtWork wk;
-  wk.bin_of = (uint8_t*)c->bin_of.p;
-  wk.desc = (ExtDesc*)c->desc.p;
-  wk.rec = (int32_t*)c->rec.p;
-  wk.counts = (uint32_t*)c->small.p;
+  wk.bin_of = (uint8_t*)c->ws->bin_of.p;
+  wk.desc = (ExtDesc*)c->ws->desc.p;
+  wk.rec = (int32_t*)c->ws->rec.p;
+  wk.counts = (uint32_t*)c->ws->small.p;
   wk.cursor = wk.counts + kCountSlots;
   wk.err = (int32_t*)(wk.cursor + kCountSlots);
   wk.fetch = (uint32_t*)(wk.err + 4);
-  wk.nbin_of = (uint8_t*)c->nbin_of.p;
-  wk.need_plain = (uint8_t*)c->need_plain.p;
+  wk.nbin_of = (uint8_t*)c->ws->nbin_of.p;
+  wk.need_plain = (uint8_t*)c->ws->need_plain.p;
   return wk;
 }
 
@@ -255,12 +266,12 @@ static int ext_args(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, E
 }
 
 static int ext_buffers(bsx_ctx* c, size_t n) {
-  HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
-  HIPCHK(c, c->desc.ensure(sizeof(ExtDesc) * (n + 1)), "hipMalloc(desc)");
-  HIPCHK(c, c->rec.ensure(32 * (n + 1)), "hipMalloc(rec)");
-  HIPCHK(c, c->small.ensure(kSmallBytes), "hipMalloc(work)");
-  HIPCHK(c, c->nbin_of.ensure(n + 1), "hipMalloc(nbin_of)");
-  HIPCHK(c, c->need_plain.ensure(n + 1), "hipMalloc(need_plain)");
+  HIPCHK(c, c->ws->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
+  HIPCHK(c, c->ws->desc.ensure(sizeof(ExtDesc) * (n + 1)), "hipMalloc(desc)");
+  HIPCHK(c, c->ws->rec.ensure(32 * (n + 1)), "hipMalloc(rec)");
+  HIPCHK(c, c->ws->small.ensure(kSmallBytes), "hipMalloc(work)");
+  HIPCHK(c, c->ws->nbin_of.ensure(n + 1), "hipMalloc(nbin_of)");
+  HIPCHK(c, c->ws->need_plain.ensure(n + 1), "hipMalloc(need_plain)");
   return BSX_OK;
 }
 
@@ -286,11 +297,11 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   BandPlan plan{kBandPlain, 0, nullptr, false, false};
   if (c->band_policy == BSX_BAND_LEARNED) {
     if (!c->model_ready) return fail(c, BSX_EINVAL, "learned band policy without a band model (bsx_band_model_set)");
-    HIPCHK(c, c->cls.ensure(n + 1), "hipMalloc(cls)");
+    HIPCHK(c, c->ws->cls.ensure(n + 1), "hipMalloc(cls)");
     time_begin(wk, kTimedSlots - 1, c->stream);
-    HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), (uint8_t*)c->cls.p, c->num_cu, c->stream), "band predict");
+    HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), (uint8_t*)c->ws->cls.p, c->num_cu, c->stream), "band predict");
     time_end(wk, kTimedSlots - 1, c->stream);
-    plan = BandPlan{kBandLearned, 0, (const uint8_t*)c->cls.p, false, false};
+    plan = BandPlan{kBandLearned, 0, (const uint8_t*)c->ws->cls.p, false, false};
   } else if (c->band_policy == BSX_BAND_FIXED) {
     plan = BandPlan{kBandFixed, c->band_class, nullptr, true, false};
   }
@@ -527,7 +538,7 @@ int bsx_extend_batch(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, 
   std::vector<uint8_t> ok;
   if (rejected > 0) {  // copy back only the tasks that ran; rejected slots stay untouched
     ok.resize(n);
-    HIPCHK(c, hipMemcpy(ok.data(), c->bin_of.p, n, hipMemcpyDeviceToHost), "D2H bin_of");
+    HIPCHK(c, hipMemcpy(ok.data(), c->ws->bin_of.p, n, hipMemcpyDeviceToHost), "D2H bin_of");
   }
   std::vector<int32_t> tmp;
   std::vector<int64_t> tmp64;
@@ -630,21 +641,21 @@ int bsx_global_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tas
   a.target = t->target; a.t_off = t->t_off; a.tlen = t->tlen; a.w = t->w;
   a.score = r->score; a.n_cigar = r->n_cigar; a.cigar = r->cigar; a.cigar_slot = r->cigar_slot;
   const size_t n = (size_t)t->n;
-  HIPCHK(c, c->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
-  HIPCHK(c, c->list.ensure(4 * n + 4), "hipMalloc(list)");
-  HIPCHK(c, c->small.ensure(kSmallBytes), "hipMalloc(work)");
+  HIPCHK(c, c->ws->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
+  HIPCHK(c, c->ws->list.ensure(4 * n + 4), "hipMalloc(list)");
+  HIPCHK(c, c->ws->small.ensure(kSmallBytes), "hipMalloc(work)");
   GlbWork wk;
-  wk.bin_of = (uint8_t*)c->bin_of.p;
-  wk.list = (int32_t*)c->list.p;
-  wk.counts = (uint32_t*)c->small.p;
+  wk.bin_of = (uint8_t*)c->ws->bin_of.p;
+  wk.list = (int32_t*)c->ws->list.p;
+  wk.counts = (uint32_t*)c->ws->small.p;
   wk.cursor = wk.counts + kCountSlots;
   wk.err = (int32_t*)(wk.cursor + kCountSlots);
   wk.fetch = (uint32_t*)(wk.err + 4);
   wk.zslot = c->zslot;
   wk.z = nullptr;
   if (r->cigar) {
-    HIPCHK(c, c->zbuf.ensure((size_t)glb_groups(c->num_cu) * (size_t)c->zslot), "hipMalloc(z)");
-    wk.z = (uint8_t*)c->zbuf.p;
+    HIPCHK(c, c->ws->zbuf.ensure((size_t)glb_groups(c->num_cu) * (size_t)c->zslot), "hipMalloc(z)");
+    wk.z = (uint8_t*)c->ws->zbuf.p;
   }
   HIPCHK(c, launch_global_impl(a, wk, c->num_cu, c->stream), "global launch");
   HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
@@ -700,7 +711,7 @@ int bsx_global_batch(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t, 
   c->pending_check = false;
   const int rejected = c->err_host[0], overflow = c->err_host[2];
   std::vector<uint8_t> ok(n, 0);
-  if (rejected > 0) HIPCHK(c, hipMemcpy(ok.data(), c->bin_of.p, n, hipMemcpyDeviceToHost), "D2H bin_of");
+  if (rejected > 0) HIPCHK(c, hipMemcpy(ok.data(), c->ws->bin_of.p, n, hipMemcpyDeviceToHost), "D2H bin_of");
   std::vector<int32_t> sco(n), ncg(n);
   std::vector<uint32_t> cg(n * slot);
   if (r->score) HIPCHK(c, hipMemcpy(sco.data(), drs.score, 4 * n, hipMemcpyDeviceToHost), "D2H score");
@@ -717,6 +728,109 @@ int bsx_global_batch(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t, 
                 c->err_host[1]);
   if (overflow > 0) return fail(c, BSX_ERANGE, "%d CIGAR(s) did not fit their slot", overflow);
   return BSX_OK;
+}
+
+}  // extern "C"
+
+struct bsx_graph {
+  bsx_ctx* ctx = nullptr;
+  Work work;                  // the graph's own workspace
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int kind = 0;               // 1 = extension, 2 = global
+};
+
+// Captures one device-memory batch call on the context's stream into a graph
+// that uses its own workspace; `sized` grows that workspace beforehand, since
+// no allocation may happen while the stream is captured.
+template <class Sized, class Call>
+static int capture(bsx_ctx* c, int kind, Sized sized, Call call, bsx_graph** out) {
+  if (!c || !out) return fail(c, BSX_EINVAL, "NULL argument");
+  *out = nullptr;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  bsx_graph* g = new bsx_graph();
+  g->ctx = c;
+  g->kind = kind;
+  const int kt_calls = c->kt_calls;
+  c->kt_calls = 0;  // no per-kernel timing events inside a graph
+  c->ws = &g->work;
+  int rc = sized();
+  if (rc == BSX_OK) {
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+      rc = hip_fail(c, e, "hipStreamBeginCapture");
+    } else {
+      rc = call();
+      hipGraph_t gr = nullptr;
+      e = hipStreamEndCapture(c->stream, &gr);
+      g->graph = gr;
+      if (rc == BSX_OK && e != hipSuccess) rc = hip_fail(c, e, "hipStreamEndCapture");
+      if (rc == BSX_OK) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(c, e, "hipGraphInstantiate");
+      }
+    }
+  }
+  c->ws = &c->wown;
+  c->kt_calls = kt_calls;
+  c->pending_check = false;
+  if (rc != BSX_OK) {
+    bsx_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return BSX_OK;
+}
+
+extern "C" {
+
+int bsx_graph_capture_extend(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, bsx_ext_results* r,
+                             bsx_graph** out) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  auto sized = [&]() -> int {
+    const size_t n = t->n > 0 ? (size_t)t->n : 0;
+    int rc = ext_buffers(c, n);
+    if (rc) return rc;
+    if (c->band_policy == BSX_BAND_LEARNED) HIPCHK(c, c->ws->cls.ensure(n + 1), "hipMalloc(cls)");
+    return BSX_OK;
+  };
+  return capture(c, 1, sized, [&]() { return bsx_extend_batch_device(c, sc, t, r); }, out);
+}
+
+int bsx_graph_capture_global(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t, bsx_glb_results* r,
+                             bsx_graph** out) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  auto sized = [&]() -> int {
+    const size_t n = t->n > 0 ? (size_t)t->n : 0;
+    HIPCHK(c, c->ws->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
+    HIPCHK(c, c->ws->list.ensure(4 * n + 4), "hipMalloc(list)");
+    HIPCHK(c, c->ws->small.ensure(kSmallBytes), "hipMalloc(work)");
+    if (r->cigar) HIPCHK(c, c->ws->zbuf.ensure((size_t)glb_groups(c->num_cu) * (size_t)c->zslot), "hipMalloc(z)");
+    return BSX_OK;
+  };
+  return capture(c, 2, sized, [&]() { return bsx_global_batch_device(c, sc, t, r); }, out);
+}
+
+int bsx_graph_launch(bsx_ctx* c, bsx_graph* g) {
+  if (!c || !g || g->ctx != c || !g->exec) return fail(c, BSX_EINVAL, "graph does not belong to this context");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipGraphLaunch(g->exec, c->stream), "hipGraphLaunch");
+  c->pending_check = true;
+  c->pending_kind = g->kind;
+  return BSX_OK;
+}
+
+void bsx_graph_destroy(bsx_graph* g) {
+  if (!g) return;
+  if (g->ctx) {
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+  }
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->work.release();
+  delete g;
 }
 
 int bsx_ksw_extend2(int qlen, const uint8_t* query, int tlen, const uint8_t* target, int m,

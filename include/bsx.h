@@ -192,6 +192,23 @@ int bsx_band_grad_device(bsx_ctx *ctx, const uint16_t *feats, const uint8_t *lab
 int bsx_band_train_device(bsx_ctx *ctx, const uint16_t *feats, const uint8_t *labels, int64_t n,
                           int64_t batch, int steps, float lr, float *losses);
 
+/* ---- graph-captured batch submission (no counterpart in the reference) ----
+ * Captures one device-memory batch call (its binning, kernels and the error
+ * read-back) into a HIP graph that keeps the task/result pointers and the
+ * batch size, and owns its own device workspace.  To stream batches, refill
+ * the same task buffers and replay with bsx_graph_launch; bsx_sync reports
+ * rejected tasks as for the eager call.  The band policy, scoring and (for
+ * CIGARs) slot size in force at capture are part of the graph; a band model
+ * set later is used by later replays.  Replays of one graph on a context are
+ * ordered on its stream. */
+typedef struct bsx_graph bsx_graph;
+int  bsx_graph_capture_extend(bsx_ctx *ctx, const bsx_scoring *sc, const bsx_ext_tasks *tasks,
+                              bsx_ext_results *res, bsx_graph **graph);
+int  bsx_graph_capture_global(bsx_ctx *ctx, const bsx_scoring *sc, const bsx_glb_tasks *tasks,
+                              bsx_glb_results *res, bsx_graph **graph);
+int  bsx_graph_launch(bsx_ctx *ctx, bsx_graph *graph);
+void bsx_graph_destroy(bsx_graph *graph);
+
 /* ---- per-kernel timing (measurement; no counterpart in the reference) ----
  * bsx_set_kernel_timing(ctx, k): the next k extension calls on the context
  * record HIP events around each kernel launch (0 switches it off).
