@@ -42,8 +42,11 @@ namespace bsx {
 
 
 
+// waves per SIMD the register budget must allow (4 keeps VGPRs <= 128)
+constexpr int plain_min_waves(int C) { return C <= 6 ? 4 : C <= 8 ? 3 : 1; }
+
 template <int L, int C, bool E1>
-__global__ void __launch_bounds__(kExtBlock)
+__global__ void __launch_bounds__(kExtBlock, plain_min_waves(C))
 ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
                  int bin) {
   __shared__ uint32_t lds_next;
