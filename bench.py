@@ -93,8 +93,8 @@ def learned_leg(args, eng, ptrs, outs, cells_d, plain_out, plain_cells, n, total
             st = eng.band_stats()
         finally:
             eng.set_band_policy(bsx.BAND_PLAIN)
-        routed = {w: int(st[c, 0]) for c, w in enumerate(("w8", "w16", "w32", "w64", "own"))}
-        rejected = {w: int(st[c, 1]) for c, w in enumerate(("w8", "w16", "w32", "w64"))}
+        routed = {w: int(st[c, 0]) for c, w in enumerate(("f24", "f40", "f80", "f144", "plain"))}
+        rejected = {w: int(st[c, 1]) for c, w in enumerate(("f24", "f40", "f80", "f144"))}
         narrow = int(st[:4, 0].sum())
         entry = {"value": n / (ms / 1e3), "unit": "extensions/s", "ms_per_step": ms,
                  "gcups": total_cells / (ms / 1e3) / 1e9, "mismatches_vs_plain": mism,

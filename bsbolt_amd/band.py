@@ -1,11 +1,12 @@
 """Band predictor of the learned-band form: training on the GPU and model files.
 
 The predictor is an MLP 24 -> 64 -> 64 -> 5 over per-task features; its
-classes are the narrow bands w = 8, 16, 32, 64 and "the task's own band".
-Labels are the smallest class whose narrow run the exact certificate accepts
-(computed on the GPU by bsx_band_labels_device), so a task the predictor sends
-to too narrow a band costs a rejected narrow attempt plus the plain run, and
-one sent to too wide a band costs the difference in frame width.  Training,
+classes are band-following frames of 24, 40, 80 and 144 columns and "plain"
+(the task's own band in the plain kernel).  Labels are the smallest frame that
+holds the task's whole run (computed on the GPU by bsx_band_labels_device), so
+a task the predictor sends to too small a frame costs a failed frame attempt
+plus the plain run, and one sent to too large a frame costs the difference in
+frame width.  Training,
 inference and labelling all run through libbsx.so (hand-written HIP kernels);
 this module only moves arrays and logs.
 
@@ -133,7 +134,7 @@ def main(argv=None):
     wl.close()
     log(f"{args.workload}: {len(batch)} tasks in {time.time() - t0:.1f}s")
     X, y = features_and_labels(eng, batch)
-    log("label histogram (w8, w16, w32, w64, own): " + str([int((y == c).sum()) for c in range(5)]))
+    log("label histogram (f24, f40, f80, f144, plain): " + str([int((y == c).sum()) for c in range(5)]))
     nval = min(200_000, len(y) // 10)
     params, info = train(eng, X[nval:], y[nval:], steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed,
                          log=log, val=(X[:nval], y[:nval]))

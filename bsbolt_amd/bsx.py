@@ -41,8 +41,8 @@ EXT_SHAPES = ((16, 1), (16, 2), (16, 3), (16, 4), (16, 6), (16, 8), (32, 6), (32
 
 # learned-band form (include/bsx.h)
 BAND_PLAIN, BAND_LEARNED, BAND_FIXED = 0, 1, 2
-BAND_CLASSES = 5              # classes 0..3 = w 8/16/32/64, 4 = the task's own band
-BAND_WIDTHS = (8, 16, 32, 64)
+BAND_CLASSES = 5              # classes 0..3 = band-following frames of 24/40/80/144 columns, 4 = plain
+BAND_FRAMES = (24, 40, 80, 144)
 BAND_FEATURES = 24
 BAND_FEATURE_STRIDE = 32
 BAND_PARAMS = 6085
@@ -327,7 +327,9 @@ class Engine:
                     g.launch()
                     self.sync()
                     score[sl], ncig[sl] = ds.numpy(), dn.numpy()
-                    cig[sl] = dc.numpy().reshape(batch, slot)
+                    # words past a task's n_cigar are left as the device buffer held them
+                    blk = dc.numpy().reshape(batch, slot)
+                    cig[sl] = np.where(np.arange(slot)[None, :] < ncig[sl][:, None], blk, 0)
             finally:
                 g.close()
         if full < n:
@@ -391,7 +393,7 @@ class Engine:
         return f.numpy()
 
     def band_labels(self, tasks: ExtTasks, scoring: Scoring = None):
-        """Smallest band class the certificate accepts per task (4: none)."""
+        """Smallest frame class whose band-following run holds each task (4: none)."""
         keep, dev = self.tasks_to_device(tasks)
         dev["n"] = len(tasks)
         lab = self.empty((len(tasks),), np.uint8)

@@ -27,8 +27,8 @@ constexpr BinShape kGlbShape[kGlbBins] = {
 constexpr int kExtSlices = 8;  // fetch-counter slots per CIGAR bin
 constexpr int glb_bin_limit(int b) { return kGlbShape[b].lanes * kGlbShape[b].cols; }
 
-// Learned-band form: band classes 0..3 run the narrow kernel at w = 8, 16, 32,
-// 64; class 4 is the task's own band (plain kernel).  Count slots: plain bins
+// Learned-band form: classes 0..3 run the band-following frame kernel with
+// frames of 24, 40, 80, 144 columns; class 4 is the plain kernel.  Count slots: plain bins
 // [0, kExtBins), narrow classes [kExtBins, kExtBins + kBandClasses - 1).
 constexpr int kBandClasses = 5;
 constexpr int kCountSlots = 16;

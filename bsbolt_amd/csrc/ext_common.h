@@ -5,10 +5,14 @@
 namespace bsx {
 
 // ---- DPP group primitives (identity 0, values >= 0) ----
-// A group is L = 8, 16, 32 or 64 consecutive lanes of the wave.
+// A group is L = 4, 8, 16, 32 or 64 consecutive lanes of the wave.
 
 template <int L>
 __device__ __forceinline__ int grp_incl_max(int x) {
+  if (L == 4) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x90, 0xf, 0xf, false));  // quad_perm [0,0,1,2]
+    return max(x, __builtin_amdgcn_update_dpp(0, x, 0x40, 0xf, 0xf, false));  // quad_perm [0,0,0,1]
+  }
   if (L == 8) {
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x90, 0xf, 0xf, false));  // quad_perm [0,0,1,2]
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x40, 0xf, 0xf, false));  // quad_perm [0,0,0,1]
@@ -29,7 +33,7 @@ __device__ __forceinline__ int grp_incl_max(int x) {
 template <int L>
 __device__ __forceinline__ int grp_excl_from_incl(int incl, int lig) {
   if (L == 16) return __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
-  const int e = L == 8 ? __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true)   // row_shr:1
+  const int e = L <= 8 ? __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true)   // row_shr:1
                        : __builtin_amdgcn_update_dpp(0, incl, 0x138, 0xf, 0xf, true);  // wave_shr:1
   return lig == 0 ? 0 : e;
 }
@@ -40,6 +44,7 @@ __device__ __forceinline__ int grp_max(int x, int lane) {
   if (L <= 16) {
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true));   // quad_perm [1,0,3,2]
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true));   // quad_perm [2,3,0,1]
+    if (L == 4) return x;
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true));  // row_half_mirror
     if (L == 16) x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true));  // row_mirror
     return x;
@@ -68,6 +73,29 @@ struct ExtKArgs {
   uint32_t trow_hi[5];  // by target code t: byte 0 holds mat[t*5+4]
   int tpb;              // task descriptors owned by one block
 };
+
+// The eight result words of a task (ksw.c:473-478 plus the cell count), one
+// word per lane of the group (two per lane for L = 4).
+template <int L>
+__device__ __forceinline__ void write_rec(int32_t* rec, int task, int lig, int best, int qle, int tle, int gtle,
+                                          int gscore, int moff, uint32_t ncell) {
+  constexpr int S = L < 8 ? L : 8;
+#pragma unroll
+  for (int r = 0; r < 8; r += S) {
+    const int f = lig + r;
+    if (f < 8) {
+      int v = best;
+      v = f == 1 ? qle : v;
+      v = f == 2 ? tle : v;
+      v = f == 3 ? gtle : v;
+      v = f == 4 ? gscore : v;
+      v = f == 5 ? moff : v;
+      v = f == 6 ? (int)ncell : v;
+      v = f == 7 ? 0 : v;
+      rec[(int64_t)task * 8 + f] = v;
+    }
+  }
+}
 
 // Band of a task clamped as ksw.c:399-407, in the same double arithmetic.
 __device__ __forceinline__ int ext_clamped_w(const ExtArgs& A, int64_t t) {

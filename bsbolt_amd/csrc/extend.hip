@@ -43,7 +43,7 @@ namespace bsx {
 
 
 // waves per SIMD the register budget must allow (4 keeps VGPRs <= 128)
-constexpr int plain_min_waves(int C) { return C <= 6 ? 4 : C <= 8 ? 3 : 1; }
+constexpr int plain_min_waves(int C) { return C <= 6 ? 4 : C <= 8 ? 3 : C <= 16 ? 2 : 1; }
 
 template <int L, int C, bool E1>
 __global__ void __launch_bounds__(kExtBlock, plain_min_waves(C))
@@ -178,10 +178,7 @@ ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_
         stage1();
         load_desc();
         if (tlen <= 0) {
-          if (lig < 8) {  // ksw.c:473-478 with no row: qle = tle = gtle = 0, gscore = -1
-            const int v = lig == 0 ? h0 : lig == 4 ? -1 : 0;
-            A.rec[(int64_t)task * 8 + lig] = v;
-          }
+          write_rec<L>(A.rec, task, lig, h0, 0, 0, 0, -1, 0, 0u);  // ksw.c:473-478 with no row
         } else {
           need = false;
         }
@@ -328,17 +325,7 @@ ext_plain_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_
       if (i >= tlen) fin = true;
     }
     if (fin) {
-      if (lig < 8) {  // ksw.c:473-478, one field per lane
-        int v = best;
-        v = lig == 1 ? best_j + 1 : v;
-        v = lig == 2 ? best_i + 1 : v;
-        v = lig == 3 ? gbest_i + 1 : v;
-        v = lig == 4 ? gbest : v;
-        v = lig == 5 ? moff : v;
-        v = lig == 6 ? (int)ncell : v;
-        v = lig == 7 ? 0 : v;
-        A.rec[(int64_t)task * 8 + lig] = v;
-      }
+      write_rec<L>(A.rec, task, lig, best, best_j + 1, best_i + 1, gbest_i + 1, gbest, moff, ncell);  // ksw.c:473-478
       need = true;
     }
   }

@@ -1,5 +1,5 @@
 // Band predictor of the learned-band form: per-task features, an MLP
-// 24 -> 64 -> 64 -> 5 (ReLU, softmax over the band classes 8/16/32/64/own band)
+// 24 -> 64 -> 64 -> 5 (ReLU, softmax over the frame classes 24/40/80/144/plain)
 // with BF16 weights and FP32 accumulation on MFMA, its backward pass and Adam.
 //
 // Matrix layout.  Activations are kept transposed (features x tasks) so the

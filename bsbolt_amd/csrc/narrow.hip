@@ -1,46 +1,37 @@
-// Narrow-band ksw_extend2 with an exact acceptance certificate (learned-band form).
+// Band-following frame run of ksw_extend2 (learned-band form).
 //
-// The reference call runs at band W (the task's w clamped as ksw.c:399-407).
-// The narrow run N applies the same recurrences with the tighter cap WN < W on
-// both sides of the diagonal: beg >= i - WN, end <= i + WN + 1.  N's result is
-// returned only if it is provably identical, in all six outputs and the cell
-// count, to the W run; otherwise the task is flagged for the plain kernel.
+// The reference computes, per row, only the columns of its adaptive band
+// [beg, end) (ksw.c:412-469); for short-lived and drifting extensions that band
+// is far narrower than the query.  This kernel gives a task a frame of WF = L*C
+// consecutive columns [f0, f0 + WF) that follows the band: f0 <= beg always (f0
+// advances toward beg by at most two columns per row), and the row is computed
+// exactly as the plain kernel computes it, over the frame.  The run IS the
+// reference run -- same recurrences, same beg/end, same stale state -- as long
+// as no non-zero cell falls past the frame:
+//   when the band end reaches past the frame (end > f0 + WF - 1 = eF), the
+//   cells in [eF, end) must all be 0.  That holds, and is checked, when the
+//   frame's last column eF computes H(i, eF) = 0 and the columns past it hold
+//   zero state: their first-row insertion value max(h0 - oe_ins - (j-1)*e_ins, 0)
+//   is 0 at j = eF + 1 (ksw.c:395-397), and any earlier row that reached past
+//   the frame wrote only zeros there (this same check).
+// A run that fails the check is flagged for the plain kernel; every accepted
+// result equals the reference's in all six outputs and the cell count.
 //
-// Certificate.  Claim: W's cells outside N's band are all zero, so W and N hold
-// the same eh[] state wherever N reads it and derive the same beg/end.  It holds
-// at row 0 and is kept, row by row, by three O(1) checks:
-//   (R1) when W's end lies beyond N's, the first column past N's end has no
-//        first-row insertion value left: max(h0 - oe_ins - (j-1)*e_ins, 0) == 0
-//        at j = N's end (ksw.c:395-397).  Columns past it were never written by
-//        N, so W reads only zeros there;
-//   (R2) no insertion value leaves N's band to the right: F at N's end is 0;
-//   (L)  no non-zero column is dropped on the left: the column that leaves N's
-//        band at the next row (j = i - WN) is not the first non-zero column of
-//        the row, which W's zero scan (ksw.c:466-467) would keep in its band.
-// Under the claim W's row maximum, z-drop and zero scans see N's values, and
-// W's band is beg_W = max(beg, i-W), end_W = min(end, i+W+1, qlen) with N's
-// beg/end.  What W computes beyond N is zeros: the cell count uses W's band,
-// and the end-to-end column (ksw.c:450-453) reads H(i, qlen-1), which is N's
-// value when qlen-1 is in N's band and 0 otherwise.
-//
-// Layout.  Diagonal frame: frame position k = g*C + c (lane g, column c) holds
-// column j = i - WN + k at row i, so a task costs WF = L*C >= 2*WN+2 positions
-// per row instead of its query length.  G (= H(i-1, j-1)) and E stay in place
-// during a row exactly as in the plain kernel; after the row the frame moves one
-// column right (G and E shift one position left) and the entering column starts
-// from its first-row value.  Positions left of column 0 and dead columns (left of
-// beg) hold G = E = 0, which makes their H zero and their F contribution <= 0, so
-// no mask against them is needed.  F keys are taken relative to the frame
-// ((k+1)*e_ins instead of (j+1)*e_ins): the offset is the same for every column
-// of a row and cancels.  The query profile is staged per task in LDS, indexed by
-// j + WN, so frame position k of row i reads entry i + k.
+// Layout: frame position k = g*C + c (lane g, column c) holds column f0 + k.
+// G (= H(i-1, j-1)) and E stay in place during a row; when the frame advances,
+// G and E shift one position left and the entering column takes its first-row
+// value (it was never computed, or computed as 0 where that value is 0).  Dead
+// columns left of beg hold G = E = 0, so they need no masks.  F keys are taken
+// relative to the frame ((k+1)*e_ins); the offset is common to the row and
+// cancels.  Raw query codes are staged per task in LDS; each row selects its
+// target base's score row (mat[t*5+q]) and extracts per-column scores with
+// v_perm_b32.
 #include "bsx_internal.h"
 #include "ext_common.h"
 
 namespace bsx {
 
-__host__ __device__ constexpr int band_of_class(int c) { return 8 << c; }  // 8, 16, 32, 64
-__host__ __device__ constexpr int frame_lanes(int c) { return c <= 1 ? 8 : 16; }
+__host__ __device__ constexpr int frame_lanes(int c) { return c <= 1 ? 8 : 16; }  // frames 24, 40, 80, 144
 __host__ __device__ constexpr int frame_cols(int c) { return c == 0 ? 3 : c == 1 ? 5 : c == 2 ? 5 : 9; }
 __host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) * frame_cols(c); }
 
@@ -82,19 +73,18 @@ __device__ __forceinline__ int grp_or(int x, int lane) {
   return lane < 32 ? a : b;
 }
 
-template <int L, int C, int WN, bool E1>
+template <int L, int C, bool E1>
 __global__ void __launch_bounds__(kExtBlock)
 ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
                   int cls, uint8_t* __restrict__ need_plain) {
   constexpr int WF = L * C;
-  static_assert(WF >= 2 * WN + 2, "the frame must hold the band and the column at its end");
   constexpr int NG = kExtBlock / L;
   constexpr int QD = (kNarrowQMax + 6) / 4;  // aligned dwords that can cover a query
   constexpr int QR = (QD + L - 1) / L;       // query dword loads per lane
-  // Query bytes of a group: byte WN + sh + j holds query[j] (sh = the query's
-  // offset in its first aligned dword); rows read bytes sh + i + k <= 3 + qlen + WN + WF - 1.
-  constexpr int SZW_STAGE = WN / 4 + L * QR;
-  constexpr int SZW_READ = (WN + WF + kNarrowQMax + 3) / 4 + 1;
+  // Query bytes of a group: byte sh + j holds query[j] (sh = the query's offset
+  // in its first aligned dword); rows read bytes sh + f0 + k <= 3 + qlen + WF - 1.
+  constexpr int SZW_STAGE = L * QR;
+  constexpr int SZW_READ = (WF + kNarrowQMax + 3) / 4 + 1;
   constexpr int SZW = SZW_STAGE > SZW_READ ? SZW_STAGE : SZW_READ;  // dwords per group
   __shared__ uint32_t q_lds[NG * SZW];
   __shared__ uint32_t lds_next;
@@ -164,7 +154,7 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
   for (int c = 0; c < C; ++c) { G[c] = 0; E[c] = 0; }
 #pragma unroll
   for (int r = 0; r < QR; ++r) nx_qd[r] = 0;
-  int task = 0, qlen = 0, tlen = 0, h0 = 0, W = 0, i = 0, ba = 0, ea = 0;
+  int task = 0, qlen = 0, tlen = 0, h0 = 0, W = 0, i = 0, ba = 0, ea = 0, f0 = 0;
   int best = 0, best_i = -1, best_j = -1, gbest = -1, gbest_i = -1, moff = 0;
   uint32_t ncell = 0, tch = 0;
   uint32_t tnx[4] = {0, 0, 0, 0};
@@ -184,15 +174,15 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
         task = nx_task; qlen = nx_qlen; tlen = nx_tlen; h0 = nx_h0; W = nx_w;
         // raw query dwords; bytes outside the query land only in columns that are never live
 #pragma unroll
-        for (int r = 0; r < QR; ++r) Q32[WN / 4 + lig + L * r] = nx_qd[r];
+        for (int r = 0; r < QR; ++r) Q32[lig + L * r] = nx_qd[r];
         sh = nx_sh;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int c = 0; c < C; ++c) {  // row-0 frame: column j = k - WN (ksw.c:395-397)
-          const int j = base + c - WN;
-          G[c] = (j >= 0 && j <= qlen) ? ramp_at(j, h0, oe_ins, e_ins) : 0;
+        for (int c = 0; c < C; ++c) {  // first frame: column j = k, first-row values (ksw.c:395-397)
+          const int j = base + c;
+          G[c] = j <= qlen ? ramp_at(j, h0, oe_ins, e_ins) : 0;
           E[c] = 0;
         }
         {
@@ -202,7 +192,7 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
           tch = v;
         }
         tcb = 0;
-        i = 0; ba = 0; ea = qlen;
+        i = 0; ba = 0; ea = qlen; f0 = 0;
         best = h0; best_i = best_j = -1; gbest = -1; gbest_i = -1; moff = 0;
         ncell = 0;
         tbase = tlen > 0 ? A.target + nx_toff : A.query + nx_qoff;
@@ -210,10 +200,7 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
         stage1();
         load_desc();
         if (tlen <= 0) {
-          if (lig < 8) {
-            const int v = lig == 0 ? h0 : lig == 4 ? -1 : 0;
-            A.rec[(int64_t)task * 8 + lig] = v;
-          }
+          write_rec<L>(A.rec, task, lig, h0, 0, 0, 0, -1, 0, 0u);  // ksw.c:473-478 with no row
         } else {
           need = false;
         }
@@ -241,12 +228,20 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
       s_hi = t == k2 ? A.trow_hi[k2] : s_hi;
     }
 
-    const int j0 = i - WN;  // column of frame position 0
-    const int bN = max(ba, j0), bW = max(ba, i - W);
-    const int eN = min(min(ea, i + WN + 1), qlen), eW = min(min(ea, i + W + 1), qlen);
-    const int kN = eN - j0;  // frame position of N's end
-    const int k0 = -j0;      // frame position of column 0
-    bool fail = eW > eN && ramp_at(eN, h0, oe_ins, e_ins) > 0;  // (R1)
+    // the reference's band of this row (ksw.c:413-416)
+    if (ba < i - W) {  // the band cap moves beg: zero the frame columns it passes (dead columns)
+      ba = i - W;
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        if (f0 + base + c < ba) { G[c] = 0; E[c] = 0; }
+    }
+    if (ea > i + W + 1) ea = i + W + 1;
+    if (ea > qlen) ea = qlen;
+    const int bN = ba, eN = ea;
+    const int eF = min(eN, f0 + WF - 1);  // band end computed in the frame
+    const int kN = eF - f0;               // frame position of eF
+    const bool spill = eN > eF;           // the band reaches past the frame: those cells must be 0
+    bool fail = spill && ramp_at(eF + 1, h0, oe_ins, e_ins) > 0;
     int h1p = 0;
     if (bN == 0) {
       h1p = h0 - (A.o_del + e_del * (i + 1));
@@ -254,11 +249,10 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
     }
 
     int M[C], ex[C];
-    // rows past qlen + WN never run (their band is empty, ksw.c:460), so the clamp never binds
-    const uint8_t* __restrict__ Qr = Q8 + sh + min(i, WN + kNarrowQMax);
+    const uint8_t* __restrict__ Qr = Q8 + sh + f0;  // f0 <= beg <= qlen
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const uint32_t qc = Qr[base + c];  // query code of column j = i - WN + k
+      const uint32_t qc = Qr[base + c];  // query code of column f0 + k
       M[c] = (int)(int8_t)(__builtin_amdgcn_perm(s_hi, s_lo, qc) & 0xffu);
     }
     // frame positions k < kN are live; the other band masks derive from these compares
@@ -292,50 +286,45 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
       const int ne = max(max(E[c] - e_del, M[c] - oe_del), 0);
       mk = lt[c] ? max(mk, (hh << 11) | k) : mk;
       E[c] = lt[c] ? ne : E[c];
-      hf = (lt[c] && !lt[c + 1]) ? (hf | (hh << 1)) : hf;  // k == kN - 1
+      hf = (lt[c] && !lt[c + 1]) ? (hf | (hh << 1)) : hf;  // k == kN - 1: H(i, eF - 1)
       const bool le = c == 0 ? le0 : lt[c - 1];
-      hf = (le && !lt[c] && f > 0) ? (hf | 1) : hf;  // k == kN: F(eN) > 0, an insertion leaves the band
+      hf = (le && !lt[c] && hh != 0) ? (hf | 1) : hf;  // k == kN: H(i, eF) (must be 0 when the band spills)
     }
     const int hsh = __builtin_amdgcn_update_dpp(0, h[C - 1], L == 16 ? 0x111 : 0x138, 0xf, 0xf, true);
     uint32_t nzb = 0;  // bit c: position c holds a non-zero G or E after the row
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int k = base + c;
       int left = c == 0 ? (lig == 0 ? 0 : hsh) : h[c - 1];
-      left = k == k0 ? h1p : left;
+      left = (f0 == 0 && base + c == 0) ? h1p : left;  // column 0 takes the first-column value
       const bool le = c == 0 ? le0 : lt[c - 1];
       G[c] = le ? left : G[c];
       E[c] = (le && !lt[c]) ? 0 : E[c];
       nzb |= (uint32_t)min(G[c] | E[c], 1) << c;
     }
-    nzb &= dN >= C ? 0xffffffffu : dN > 0 ? (1u << dN) - 1u : 0u;
+    // zero scans (ksw.c:466-469): the end scan covers positions <= kN (position kN holds
+    // H(i, eF-1) in G); the beg scan stops before the band end, which is kN unless the band spills
+    auto upto = [&](int d) -> uint32_t { return d >= C ? 0xffffffffu : d > 0 ? (1u << d) - 1u : 0u; };
+    const uint32_t nz_hz = nzb & upto(dN + 1);
+    const uint32_t nz_nb = nzb & upto(spill ? dN + 1 : dN);
     int hik = 0, lok = 0;
-    if (nzb) {
-      hik = base + 32 - __builtin_clz(nzb);
-      lok = 2048 - base - __builtin_ctz(nzb);
-    }
+    if (nz_hz) hik = base + 32 - __builtin_clz(nz_hz);
+    if (nz_nb) lok = 2048 - base - __builtin_ctz(nz_nb);
     mk = grp_max<L>(mk, lane);
     hf = grp_or<L>(hf, lane);
     hik = grp_max<L>(hik, lane);
     lok = grp_max<L>(lok, lane);
-    if (eW > eN && eN > bN && (hf & 1)) fail = true;  // (R2)
+    if (spill && (hf & 1)) fail = true;  // a non-zero cell past the frame
 
-    // W's view of the row: cell count and end-to-end column (ksw.c:449-453)
-    const int h1N = eN > bN ? (hf >> 1) : h1p;
-    int h1pW = 0;
-    if (bW == 0) {
-      h1pW = h0 - (A.o_del + e_del * (i + 1));
-      h1pW = h1pW > 0 ? h1pW : 0;
+    // end-to-end column and cell count (ksw.c:449-453, 424)
+    const int h1 = eN > bN ? (spill ? 0 : (hf >> 1)) : h1p;
+    const int jafter = eN > bN ? eN : bN;
+    if (jafter == qlen) {
+      if (!(gbest > h1)) gbest_i = i;
+      gbest = gbest > h1 ? gbest : h1;
     }
-    const int h1W = eW > bW ? ((eW == eN && eN > bN) ? h1N : 0) : h1pW;
-    const int jW = eW > bW ? eW : bW;
-    if (jW == qlen) {
-      if (!(gbest > h1W)) gbest_i = i;
-      gbest = gbest > h1W ? gbest : h1W;
-    }
-    ncell += eW > bW ? (uint32_t)(eW - bW) : 0u;
+    ncell += eN > bN ? (uint32_t)(eN - bN) : 0u;
     const int m = eN > bN ? mk >> 11 : 0;
-    const int mj = (mk & 2047) + j0;
+    const int mj = (mk & 2047) + f0;
     bool fin = false;
     if (!fail) {
       if (m == 0) {
@@ -349,41 +338,38 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
         const int drop = di > dj ? best - m - (di - dj) * e_del : best - m - (dj - di) * e_ins;
         if (drop > A.zdrop) fin = true;
       }
-      if (!fin) {  // ksw.c:466-469 on N's values
-        const int nb = lok > 0 ? 2048 - lok + j0 : eN;
-        const int hz = h1N != 0 ? eN : (hik > 0 ? hik - 1 + j0 : -1);
+      if (!fin) {  // ksw.c:466-469
+        const int nb = lok > 0 ? 2048 - lok + f0 : eN;
+        const int hz = hik > 0 ? hik - 1 + f0 : -1;
         ba = nb;
-        const int ne = hz >= 0 ? hz + 2 : nb + 1;
+        const int ne = hz >= nb ? hz + 2 : nb + 1;
         ea = ne < qlen ? ne : qlen;
         ++i;
         if (i >= tlen) fin = true;
-        else if (nb == j0) fail = true;  // (L): column i-1-WN leaves N's band but is W's first non-zero
-        // move the frame one column right; the entering column has its first-row value
-        const int jin = i - WN + WF - 1;
-        const int gin = jin <= qlen ? ramp_at(jin, h0, oe_ins, e_ins) : 0;
-        const int gnx = __builtin_amdgcn_update_dpp(0, G[0], L == 16 ? 0x101 : 0x130, 0xf, 0xf, true);
-        const int enx = __builtin_amdgcn_update_dpp(0, E[0], L == 16 ? 0x101 : 0x130, 0xf, 0xf, true);
+        // move the frame toward beg, at most two columns per row; an entering column has its
+        // first-row value (it was never computed, or computed as 0 where that value is 0)
+        const int s = min(ba - f0, 2);
 #pragma unroll
-        for (int c = 0; c < C - 1; ++c) { G[c] = G[c + 1]; E[c] = E[c + 1]; }
-        G[C - 1] = lig == L - 1 ? gin : gnx;
-        E[C - 1] = lig == L - 1 ? 0 : enx;
+        for (int r = 0; r < 2; ++r) {
+          if (r < s) {
+            ++f0;
+            const int jin = f0 + WF - 1;
+            const int gin = jin <= qlen ? ramp_at(jin, h0, oe_ins, e_ins) : 0;
+            const int gnx = __builtin_amdgcn_update_dpp(0, G[0], L == 16 ? 0x101 : 0x130, 0xf, 0xf, true);
+            const int enx = __builtin_amdgcn_update_dpp(0, E[0], L == 16 ? 0x101 : 0x130, 0xf, 0xf, true);
+#pragma unroll
+            for (int c = 0; c < C - 1; ++c) { G[c] = G[c + 1]; E[c] = E[c + 1]; }
+            G[C - 1] = lig == L - 1 ? gin : gnx;
+            E[C - 1] = lig == L - 1 ? 0 : enx;
+          }
+        }
       }
     }
     if (fail) {
       if (lig == 0) need_plain[task] = 1;
       need = true;
     } else if (fin) {
-      if (lig < 8) {  // ksw.c:473-478, one field per lane
-        int v = best;
-        v = lig == 1 ? best_j + 1 : v;
-        v = lig == 2 ? best_i + 1 : v;
-        v = lig == 3 ? gbest_i + 1 : v;
-        v = lig == 4 ? gbest : v;
-        v = lig == 5 ? moff : v;
-        v = lig == 6 ? (int)ncell : v;
-        v = lig == 7 ? 0 : v;
-        A.rec[(int64_t)task * 8 + lig] = v;
-      }
+      write_rec<L>(A.rec, task, lig, best, best_j + 1, best_i + 1, gbest_i + 1, gbest, moff, ncell);  // ksw.c:473-478
       need = true;
     }
   }
@@ -400,10 +386,10 @@ __device__ __forceinline__ bool narrow_pays_dev(int c, int b) {
   return r;
 }
 
-// Narrow class of every valid task (0xff: plain only) and the class counts.  A
-// task goes narrow when its class is narrow, its query fits the narrow kernels,
-// its clamped band is wider than the class band and (unless any_gain) the class
-// frame is narrower than the plain kernel's columns for its query.
+// Frame class of every valid task (0xff: plain only) and the class counts.  A
+// task goes to a frame when its class is a frame class, its query fits the
+// frame kernels and (unless any_gain) the class's row cost is below the plain
+// kernel's for its query length.
 __global__ void __launch_bounds__(256)
 ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
   __shared__ uint32_t hist[kBandClasses];
@@ -415,8 +401,7 @@ ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
     if (b == 0xff) continue;
     const int c = plan.mode == kBandFixed ? plan.fixed : (int)plan.cls[t];
     int nb = 0xff;
-    if (c >= 0 && c < kBandClasses - 1 && A.qlen[t] <= kNarrowQMax &&
-        ext_clamped_w(A, t) > band_of_class(c) && (plan.any_gain || narrow_pays_dev(c, b)))
+    if (c >= 0 && c < kBandClasses - 1 && A.qlen[t] <= kNarrowQMax && (plan.any_gain || narrow_pays_dev(c, b)))
       nb = c;
     W.nbin_of[t] = (uint8_t)nb;
     W.need_plain[t] = nb == 0xff ? 1 : 0;
@@ -458,11 +443,11 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
 
 template <int CL, bool E1>
 static void launch_narrow_class(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st) {
-  constexpr int L = frame_lanes(CL), C = frame_cols(CL), WN = band_of_class(CL);
+  constexpr int L = frame_lanes(CL), C = frame_cols(CL);
   const int64_t blocks = (n + k.tpb - 1) / k.tpb;
   if (blocks < 1) return;
   time_begin(wk, kExtBins + CL, st);
-  hipLaunchKernelGGL((ext_narrow_kernel<L, C, WN, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
+  hipLaunchKernelGGL((ext_narrow_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, CL, wk.need_plain);
   time_end(wk, kExtBins + CL, st);
 }
@@ -500,10 +485,9 @@ hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk,
 
 // ---- labels and statistics ----
 
-// After a fixed-class run of class c: a task whose narrow result was accepted,
-// or whose clamped band is not wider than the class band (the narrow run is
-// then the reference run), gets label c.  Classes are run from 3 down to 0, so
-// the label ends as the smallest accepted class; kBandClasses-1 = none.
+// After a fixed-class run of class c: a task whose frame run completed gets
+// label c.  Classes are run from 3 down to 0, so the label ends as the
+// smallest frame that holds the task's band; kBandClasses-1 = none.
 __global__ void __launch_bounds__(256)
 band_label_merge_kernel(ExtArgs A, ExtWork W, int c, uint8_t* __restrict__ labels, int first) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
@@ -513,9 +497,7 @@ band_label_merge_kernel(ExtArgs A, ExtWork W, int c, uint8_t* __restrict__ label
       continue;
     }
     uint8_t lab = first ? (uint8_t)(kBandClasses - 1) : labels[t];
-    const bool trivial = A.qlen[t] <= kNarrowQMax && ext_clamped_w(A, t) <= band_of_class(c);
-    const bool ok = W.nbin_of[t] == c && W.need_plain[t] == 0;
-    if (trivial || ok) lab = (uint8_t)c;
+    if (W.nbin_of[t] == c && W.need_plain[t] == 0) lab = (uint8_t)c;
     labels[t] = lab;
   }
 }

@@ -148,33 +148,33 @@ int   bsx_timer_start(bsx_ctx *ctx);
 int   bsx_timer_stop(bsx_ctx *ctx, float *ms);
 
 /* ---- learned-band form of the extension ----
- * No counterpart in the reference (which always runs ksw_extend2 at the caller's
- * w).  Under BSX_BAND_LEARNED the extension entry points first predict a narrow
- * band class per task with the band model (w = 8, 16, 32, 64, or the task's own
- * band), run the narrow-band kernel for the tasks whose class is narrow, and
- * accept a narrow result only under an exact certificate that it equals the
- * reference's result at the task's band; every other task runs the plain
- * kernel.  Results are identical to BSX_BAND_PLAIN in every case.
- * BSX_BAND_FIXED runs every task whose query is <= 256 and whose clamped band
- * is wider than band class `band_class` through that class (tests, labels). */
+ * No counterpart in the reference (which always runs ksw_extend2 over its full
+ * adaptive band).  Under BSX_BAND_LEARNED the extension entry points first
+ * predict a frame class per task with the band model (band-following frames of
+ * 24, 40, 80 or 144 columns, or plain), run the frame kernel for the tasks whose
+ * class is a frame, and keep a frame result only when the run's band never left
+ * the frame with a non-zero cell (checked exactly, per row); every other task
+ * runs the plain kernel.  Results are identical to BSX_BAND_PLAIN in every case.
+ * BSX_BAND_FIXED runs every task whose query is <= 256 bases through frame class
+ * `band_class` first (tests, labels). */
 #define BSX_BAND_PLAIN   0
 #define BSX_BAND_LEARNED 1
 #define BSX_BAND_FIXED   2
-#define BSX_BAND_CLASSES 5      /* classes 0..3 = w 8/16/32/64, 4 = own band   */
+#define BSX_BAND_CLASSES 5      /* 0..3 = frames of 24/40/80/144 columns, 4 = plain */
 #define BSX_BAND_FEATURES 24    /* features per task (stored padded to 32)      */
 #define BSX_BAND_FEATURE_STRIDE 32
 #define BSX_BAND_PARAMS 6085    /* MLP 24-64-64-5: W1,b1,W2,b2,W3,b3 (row = out) */
 int bsx_set_band_policy(bsx_ctx *ctx, int policy, int band_class);
 /* Per class c of the last learned/fixed extension call: out[2c] tasks routed to
- * class c (c = 4: plain only), out[2c+1] of those rejected by the certificate. */
+ * class c (c = 4: plain only), out[2c+1] of those whose frame run fell back. */
 int bsx_band_stats(bsx_ctx *ctx, int64_t out[2 * BSX_BAND_CLASSES]);
 /* FP32 master parameters of the band model (set resets the Adam state). */
 int bsx_band_model_set(bsx_ctx *ctx, const float *params, int64_t n_params);
 int bsx_band_model_get(bsx_ctx *ctx, float *params, int64_t n_params);
 /* Device-memory forms (queued on the context's stream):
  *  features: feats[n][32] BF16 bit patterns (24 used, rest 0);
- *  labels:   labels[n] = smallest class whose narrow result the certificate
- *            accepts, 4 if none, 0xff for a task that violates a precondition;
+ *  labels:   labels[n] = smallest frame class whose run holds the task, 4 if
+ *            none, 0xff for a task that violates a precondition;
  *  forward:  logits[n][5] (nullable) and argmax class cls[n] (nullable);
  *  grad:     grad[0..6084] = mean softmax cross-entropy gradient over the n
  *            samples, grad[6085] = mean loss;

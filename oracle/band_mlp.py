@@ -11,8 +11,8 @@ by tests/ and the smoke check as the checker, never by the product path).
   grad64(params, X, y)   the same loss in float64 without rounding (the form the
                          finite-difference checks pin)
   adam(...)              one Adam step (betas 0.9/0.999, eps 1e-8)
-  labels(batch)          smallest band class whose narrow run the certificate
-                         accepts (oracle/band_oracle.c)
+  labels(batch)          smallest frame class whose band-following run is
+                         accepted (oracle/band_oracle.c)
 
 Parameter vector: W1[64][24], b1[64], W2[64][64], b2[64], W3[5][64], b3[5].
 """
@@ -29,7 +29,7 @@ NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 assert NPARAMS == 6085
 SCAN = 16
 QMAX = 256
-WIDTHS = (8, 16, 32, 64)
+FRAMES = (24, 40, 80, 144)  # columns of the frame classes 0..3
 
 
 def unpack(p):
@@ -126,9 +126,9 @@ def labels(b, scoring=None):
     return lab
 
 
-def narrow_extend(query, target, h0, wn, w=100, end_bonus=30, zdrop=100, mat=None, o_del=6, e_del=1, o_ins=6,
+def narrow_extend(query, target, h0, wf, w=100, end_bonus=30, zdrop=100, mat=None, o_del=6, e_del=1, o_ins=6,
                   e_ins=1):
-    """(accepted, outputs(6), cells, why) of one narrow run with its certificate."""
+    """(accepted, outputs(6), cells, why) of one run in a band-following frame of wf columns."""
     q = np.ascontiguousarray(query, np.uint8)
     t = np.ascontiguousarray(target, np.uint8)
     mat = _O.fill_scmat() if mat is None else np.ascontiguousarray(mat, np.int8)
@@ -136,7 +136,7 @@ def narrow_extend(query, target, h0, wn, w=100, end_bonus=30, zdrop=100, mat=Non
     cl = C.c_int64(0)
     why = C.c_int(0)
     r = _lib().orc_narrow_extend(len(q), q.ctypes.data, len(t), t.ctypes.data, mat.ctypes.data, o_del, e_del, o_ins,
-                                 e_ins, w, end_bonus, zdrop, h0, wn, out.ctypes.data, C.byref(cl), C.byref(why))
+                                 e_ins, w, end_bonus, zdrop, h0, wf, out.ctypes.data, C.byref(cl), C.byref(why))
     return r, tuple(int(x) for x in out), int(cl.value), int(why.value)
 
 

@@ -3,21 +3,21 @@
  * ONLY as the test oracle (tests/ and the smoke check).  Nothing in the product
  * library links or calls this file.
  *
- *   orc_narrow_extend   the narrow-band run of ksw_extend2 (ksw.c:380-479 with
- *                       the band capped at wn) and the exact certificate that
- *                       its six outputs and cell count equal the run at the
- *                       task's own band: the statement the GPU narrow kernel
- *                       and its labels are checked against
+ *   orc_narrow_extend   the reference ksw_extend2 run (ksw.c:380-479) watched
+ *                       through a band-following frame of WF columns: the frame
+ *                       start f0 <= beg advances toward beg by at most two
+ *                       columns per row, and the run is rejected when the band
+ *                       reaches past the frame (end > f0+WF-1 = eF) unless
+ *                       H(i, eF) = 0 and the first-row insertion value at eF+1
+ *                       (ksw.c:395-397) is 0 -- the statement the GPU frame
+ *                       kernel and its labels are checked against
+ *   orc_band_labels     smallest frame class whose run is accepted
  *   orc_band_features   the 24 per-task predictor features
  *
- * The narrow run is the restatement of ksw.c:380-479 (orc_extend2) with the cap
- * i-wn <= j <= i+wn applied to beg/end; the certificate checks, per row,
- *   left : the column that leaves the narrow band is not the first non-zero
- *          column the reference's zero scan would keep (ksw.c:466-467);
- *   ramp : the first column past the narrow end has no first-row insertion
- *          value (ksw.c:395-397) while the wide band reaches past it;
- *   F    : no insertion value leaves the narrow band to the right;
- * and replays the wide band's cell count and end-to-end column (ksw.c:449-453).
+ * The run computes the reference itself, so an accepted result is the
+ * reference's by construction.  It also checks the fact the GPU kernel relies
+ * on: every column entering the frame holds its first-row value (returns -2
+ * otherwise, which the tests treat as a failure of the argument).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -36,18 +36,24 @@ static int clampw(int w, int qlen, const int8_t *mat, int eb, int od, int ed, in
 	return w;
 }
 
-/* Returns 1 when the certificate accepts (out[0..5] = score qle tle gtle gscore
- * max_off, *cells = cells of the wide run), 0 when it rejects (*why = 1 left,
- * 2 ramp, 3 F), -1 on bad input.  wn >= the clamped band runs the wide band
- * itself (always accepted). */
+static int ramp(int j, int h0, int oe_ins, int e_ins)
+{
+	const int g = j == 0 ? h0 : h0 - oe_ins - (j - 1) * e_ins;
+	return g > 0 ? g : 0;
+}
+
+/* Returns 1 when the frame run is accepted (out[0..5] = score qle tle gtle
+ * gscore max_off, *cells = cell count), 0 when rejected (*why = 1: the band
+ * spills a non-zero cell past the frame, 2: first-row insertion value past the
+ * frame), -1 on bad input, -2 if an entering column's state is not its
+ * first-row value. */
 int orc_narrow_extend(int qlen, const uint8_t *query, int tlen, const uint8_t *target, const int8_t *mat,
                       int o_del, int e_del, int o_ins, int e_ins, int w, int end_bonus, int zdrop, int h0,
-                      int wn, int *out, int64_t *cells_out, int *why)
+                      int wf, int *out, int64_t *cells_out, int *why)
 {
-	if (qlen < 1 || h0 < 1 || tlen < 0 || wn < 0) return -1;
+	if (qlen < 1 || h0 < 1 || tlen < 0 || wf < 2) return -1;
 	const int oe_del = o_del + e_del, oe_ins = o_ins + e_ins;
-	const int W = clampw(w, qlen, mat, end_bonus, o_del, e_del, o_ins, e_ins);
-	if (wn > W) wn = W;
+	w = clampw(w, qlen, mat, end_bonus, o_del, e_del, o_ins, e_ins);
 	int32_t *hd = calloc(qlen + 2, 4), *ev = calloc(qlen + 2, 4);
 	int8_t *prof = malloc((size_t)qlen * 5 + 1);
 	for (int k = 0; k < 5; ++k)
@@ -56,30 +62,21 @@ int orc_narrow_extend(int qlen, const uint8_t *query, int tlen, const uint8_t *t
 	hd[1] = h0 > oe_ins ? h0 - oe_ins : 0;
 	for (int j = 2; j <= qlen && hd[j - 1] > e_ins; ++j) hd[j] = hd[j - 1] - e_ins;
 	int best = h0, best_i = -1, best_j = -1, gbest = -1, gbest_i = -1, moff = 0;
-	int ba = 0, ea = qlen, ok = 1;
+	int beg = 0, end = qlen, f0 = 0, ok = 1;
 	int64_t cells = 0;
 	*why = 0;
 	for (int i = 0; i < tlen; ++i) {
 		const int8_t *srow = prof + target[i] * qlen;
-		int bW = ba < i - W ? i - W : ba, eW = ea;
-		if (eW > i + W + 1) eW = i + W + 1;
-		if (eW > qlen) eW = qlen;
-		int bN = ba < i - wn ? i - wn : ba, eN = ea;
-		if (eN > i + wn + 1) eN = i + wn + 1;
-		if (eN > qlen) eN = qlen;
-		if (bN > bW) { /* left: column i-1-wn leaves the narrow band */
-			const int j = i - 1 - wn;
-			if (j >= ba && j < bN && (hd[j] != 0 || ev[j] != 0)) { ok = 0; *why = 1; break; }
-		}
-		if (eW > eN) { /* ramp past the narrow end */
-			const int j = eN;
-			const int g = j == 0 ? h0 : h0 - oe_ins - (j - 1) * e_ins;
-			if (g > 0) { ok = 0; *why = 2; break; }
-		}
+		if (beg < i - w) beg = i - w;
+		if (end > i + w + 1) end = i + w + 1;
+		if (end > qlen) end = qlen;
+		const int eF = end < f0 + wf - 1 ? end : f0 + wf - 1;
+		const int spill = end > eF;
+		if (spill && ramp(eF + 1, h0, oe_ins, e_ins) > 0) { ok = 0; *why = 2; break; }
 		int h1 = 0;
-		if (bN == 0) { h1 = h0 - (o_del + e_del * (i + 1)); if (h1 < 0) h1 = 0; }
-		int f = 0, m = 0, mj = -1;
-		for (int j = bN; j < eN; ++j) {
+		if (beg == 0) { h1 = h0 - (o_del + e_del * (i + 1)); if (h1 < 0) h1 = 0; }
+		int f = 0, m = 0, mj = -1, h_eF = 0;
+		for (int j = beg; j < end; ++j) {
 			const int M0 = hd[j];
 			int e = ev[j];
 			hd[j] = h1;
@@ -87,6 +84,7 @@ int orc_narrow_extend(int qlen, const uint8_t *query, int tlen, const uint8_t *t
 			int h = M > e ? M : e;
 			h = h > f ? h : f;
 			h1 = h;
+			if (j == eF) h_eF = h;
 			mj = m > h ? mj : j;
 			m = m > h ? m : h;
 			int t = M - oe_del; t = t > 0 ? t : 0;
@@ -95,16 +93,12 @@ int orc_narrow_extend(int qlen, const uint8_t *query, int tlen, const uint8_t *t
 			t = M - oe_ins; t = t > 0 ? t : 0;
 			f -= e_ins; f = f > t ? f : t;
 		}
-		if (eW > eN && eN > bN && f != 0) { ok = 0; *why = 3; break; } /* F leaves the band */
-		hd[eN] = h1;
-		ev[eN] = 0;
-		/* the wide run's cell count and end-to-end column */
-		cells += eW > bW ? eW - bW : 0;
-		int h1pW = 0;
-		if (bW == 0) { h1pW = h0 - (o_del + e_del * (i + 1)); if (h1pW < 0) h1pW = 0; }
-		const int h1W = eW > bW ? ((eW == eN && eN > bN) ? h1 : 0) : h1pW;
-		const int jW = eW > bW ? eW : bW;
-		if (jW == qlen) { gbest_i = gbest > h1W ? gbest_i : i; gbest = gbest > h1W ? gbest : h1W; }
+		if (spill && h_eF != 0) { ok = 0; *why = 1; break; }
+		hd[end] = h1;
+		ev[end] = 0;
+		cells += end > beg ? end - beg : 0;
+		const int jafter = end > beg ? end : beg;
+		if (jafter == qlen) { gbest_i = gbest > h1 ? gbest_i : i; gbest = gbest > h1 ? gbest : h1; }
 		if (m == 0) break;
 		if (m > best) {
 			best = m; best_i = i; best_j = mj;
@@ -115,12 +109,21 @@ int orc_narrow_extend(int qlen, const uint8_t *query, int tlen, const uint8_t *t
 			const int drop = di > dj ? best - m - (di - dj) * e_del : best - m - (dj - di) * e_ins;
 			if (drop > zdrop) break;
 		}
-		int k = bN;
-		while (k < eN && hd[k] == 0 && ev[k] == 0) ++k;
-		ba = k;
-		k = eN;
-		while (k >= ba && hd[k] == 0 && ev[k] == 0) --k;
-		ea = k + 2 < qlen ? k + 2 : qlen;
+		int k = beg;
+		while (k < end && hd[k] == 0 && ev[k] == 0) ++k;
+		beg = k;
+		k = end;
+		while (k >= beg && hd[k] == 0 && ev[k] == 0) --k;
+		end = k + 2 < qlen ? k + 2 : qlen;
+		/* the frame advances toward beg; the entering columns must hold their first-row value */
+		int s = beg - f0;
+		if (s > 2) s = 2;
+		for (int r = 0; r < s; ++r) {
+			++f0;
+			const int jin = f0 + wf - 1;
+			if (jin <= qlen && (hd[jin] != ramp(jin, h0, oe_ins, e_ins) || ev[jin] != 0)) ok = -2;
+		}
+		if (ok == -2) break;
 	}
 	free(hd);
 	free(ev);
@@ -130,23 +133,22 @@ int orc_narrow_extend(int qlen, const uint8_t *query, int tlen, const uint8_t *t
 	return ok;
 }
 
-/* Band classes 0..3 = w 8, 16, 32, 64; the label of a task is the smallest
- * class whose narrow run the certificate accepts (or whose width is not below
- * the task's clamped band), 4 if none. */
+/* Frame classes 0..3 = 24, 40, 80, 144 columns; the label of a task is the
+ * smallest class whose frame run is accepted, 4 if none (or qlen > qmax). */
 void orc_band_labels(int64_t n, const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins, int zdrop,
                      const uint8_t *query, const uint64_t *q_off, const int32_t *qlen, const uint8_t *target,
                      const uint64_t *t_off, const int32_t *tlen, const int32_t *h0, const int32_t *w,
                      const int32_t *end_bonus, int qmax, uint8_t *labels)
 {
+	static const int frames[4] = {24, 40, 80, 144};
 	for (int64_t t = 0; t < n; ++t) {
 		int lab = 4;
 		if (qlen[t] <= qmax) {
 			for (int c = 0; c < 4; ++c) {
 				int out[6], why;
 				int64_t cl;
-				const int wn = 8 << c;
 				const int r = orc_narrow_extend(qlen[t], query + q_off[t], tlen[t], target + t_off[t], mat, o_del,
-				                                e_del, o_ins, e_ins, w[t], end_bonus[t], zdrop, h0[t], wn, out,
+				                                e_del, o_ins, e_ins, w[t], end_bonus[t], zdrop, h0[t], frames[c], out,
 				                                &cl, &why);
 				if (r == 1) { lab = c; break; }
 			}

@@ -1,5 +1,5 @@
 """CPU checks of the learned-band oracle (oracle/band_oracle.c, oracle/band_mlp.py):
-the narrow-band certificate against the compiled reference, the task features
+the band-following frame run against the compiled reference, the task features
 against hand-computed values, and the MLP backward pass against finite
 differences."""
 import numpy as np
@@ -21,19 +21,21 @@ def _reference(b):
     return want
 
 
-def _check_sound(b, wn):
+def _check_sound(b, wf):
     want = _reference(b)
     acc = 0
-    whys = np.zeros(4, int)
+    whys = np.zeros(3, int)
     for i in range(len(b)):
-        r, out, cells, why = B.narrow_extend(b.query_of(i), b.target_of(i), int(b.h0[i]), wn, w=int(b.w[i]),
+        r, out, cells, why = B.narrow_extend(b.query_of(i), b.target_of(i), int(b.h0[i]), wf, w=int(b.w[i]),
                                              end_bonus=int(b.end_bonus[i]))
+        # -2: a column entered the frame without its first-row state (the argument the GPU frame relies on)
+        assert r in (0, 1), f"task {i} wf {wf}: r = {r}"
         if r == 1:
             acc += 1
             got = dict(zip(FIELDS, out))
             for k in FIELDS:
-                assert got[k] == want[k][i], f"task {i} wn {wn}: {k} {got[k]} != {want[k][i]}"
-            assert cells == want["cells"][i], f"task {i} wn {wn}: cells"
+                assert got[k] == want[k][i], f"task {i} wf {wf}: {k} {got[k]} != {want[k][i]}"
+            assert cells == want["cells"][i], f"task {i} wf {wf}: cells"
         else:
             whys[why] += 1
     return acc, whys
@@ -42,61 +44,92 @@ def _check_sound(b, wn):
 _WHYS = {}
 
 
-@pytest.mark.parametrize("wn", [8, 16, 32, 64])
-def test_certificate_sound_on_workloads(wl_small, wn):
+@pytest.mark.parametrize("wf", B.FRAMES)
+def test_frame_run_sound_on_workloads(wl_small, wf):
     tot_acc = 0
-    tot_why = np.zeros(4, int)
+    tot_why = np.zeros(3, int)
     for b in wl_small:
-        acc, whys = _check_sound(b, wn)
+        acc, whys = _check_sound(b, wf)
         tot_acc += acc
         tot_why += whys
     n = sum(len(b) for b in wl_small)
-    assert tot_acc > 0.05 * n, f"wn {wn}: accepted only {tot_acc} of {n}"
-    _WHYS[wn] = tot_why
+    assert tot_acc > 0.3 * n, f"wf {wf}: accepted only {tot_acc} of {n}"
+    _WHYS[wf] = tot_why
 
 
-def test_certificate_every_rejection_reason_occurs(wl_small):
-    if len(_WHYS) < 4:
-        for wn in (8, 16, 32, 64):
-            _WHYS[wn] = sum(_check_sound(b, wn)[1] for b in wl_small)
+def test_frame_every_rejection_reason_occurs(wl_small):
+    if len(_WHYS) < len(B.FRAMES):
+        for wf in B.FRAMES:
+            _WHYS[wf] = sum(_check_sound(b, wf)[1] for b in wl_small)
     tot = sum(_WHYS.values())
     assert all(tot[1:] > 0), _WHYS
 
 
 @pytest.mark.parametrize("seed", [11, 12])
-def test_certificate_sound_on_random_tasks(seed):
+def test_frame_run_sound_on_random_tasks(seed):
     from bsbolt_amd.tasks import random_ext_tasks
     b = random_ext_tasks(3000, seed=seed, qmax=200, h0max=120, w_choices=(16, 32, 64, 100, 200),
                          eb_choices=(0, 5, 30), n_frac=0.002)
-    for wn in (8, 16, 32, 64):
-        _check_sound(b, wn)
+    for wf in B.FRAMES:
+        _check_sound(b, wf)
 
 
-def test_certificate_full_band_is_the_reference_run():
+@pytest.mark.parametrize("seed", [21, 22])
+def test_frame_run_sound_on_random_scorings(seed):
+    """General gap extensions and asymmetric gap costs (the first-row insertion
+    values then fall by e_ins per column)."""
     from bsbolt_amd.tasks import random_ext_tasks
-    b = random_ext_tasks(500, seed=5, qmax=120, h0max=100, w_choices=(8, 16, 32), eb_choices=(0, 30))
+    b = random_ext_tasks(1500, seed=seed, qmax=200, h0max=120, w_choices=(16, 40, 100),
+                         eb_choices=(0, 5, 30), n_frac=0.01)
+    for sc in (dict(o_del=5, e_del=2, o_ins=3, e_ins=1, zdrop=50), dict(o_del=4, e_del=3, o_ins=7, e_ins=2, zdrop=0)):
+        want, _ = O.extend2_batch(b, nthreads=8, scoring=sc)
+        for i in range(len(b)):
+            for wf in (24, 80):
+                r, out, cells, why = B.narrow_extend(b.query_of(i), b.target_of(i), int(b.h0[i]), wf, w=int(b.w[i]),
+                                                     end_bonus=int(b.end_bonus[i]), zdrop=sc["zdrop"],
+                                                     o_del=sc["o_del"], e_del=sc["e_del"], o_ins=sc["o_ins"],
+                                                     e_ins=sc["e_ins"])
+                assert r in (0, 1)
+                if r == 1:
+                    assert out == tuple(int(want[k][i]) for k in FIELDS) and cells == want["cells"][i], (sc, i, wf)
+
+
+def test_frame_wider_than_query_is_the_reference_run():
+    from bsbolt_amd.tasks import random_ext_tasks
+    b = random_ext_tasks(500, seed=5, qmax=120, h0max=100, w_choices=(8, 16, 32, 200), eb_choices=(0, 30))
     want = _reference(b)
     for i in range(len(b)):
-        r, out, cells, why = B.narrow_extend(b.query_of(i), b.target_of(i), int(b.h0[i]), 64, w=int(b.w[i]),
+        r, out, cells, why = B.narrow_extend(b.query_of(i), b.target_of(i), int(b.h0[i]), 144, w=int(b.w[i]),
                                              end_bonus=int(b.end_bonus[i]))
         assert r == 1
         assert out == tuple(int(want[k][i]) for k in FIELDS) and cells == want["cells"][i]
 
 
-def test_certificate_ramp_rejection():
-    # h0 = 60: the first-row insertion values h0-7-(j-1) are still positive past column 9
+def test_frame_rejections_crafted():
     rng = np.random.default_rng(3)
-    q = rng.integers(0, 4, 50).astype(np.uint8)
-    r, out, cells, why = B.narrow_extend(q, q, 60, 8)
+    q = rng.integers(0, 4, 60).astype(np.uint8)
+    # h0 = 60: the first-row insertion values h0 - 7 - (j-1) are still positive at
+    # column 24, where the band of row 0 (the whole query) leaves a 24-column frame
+    r, out, cells, why = B.narrow_extend(q, q, 60, 24)
     assert r == 0 and why == 2
-    # h0 = 5 leaves no ramp, and a 12-base exact match keeps every cell at
-    # distance 8 from the diagonal at 0 (its deletion/insertion arms lose 6+1 per base)
-    r, out, cells, why = B.narrow_extend(q[:12], q[:12], 5, 8)
-    assert r == 1 and out[0] == 5 + 12
-    # with h0 = 14 the same match scores 26 and its insertion arm (H - 7 - d at
-    # d columns right of the diagonal) leaves the band at the end of a row
-    r, out, cells, why = B.narrow_extend(q[:12], q[:12], 14, 8)
-    assert r == 0 and why == 3
+    # ... and reach 0 at column 54, inside an 80-column frame; the exact match
+    # then keeps its band inside the frame
+    r, out, cells, why = B.narrow_extend(q, q, 60, 80)
+    assert r == 1 and out[0] == 60 + 60
+    # h0 = 5 leaves no first-row values; a 12-base exact match fits 24 columns
+    r, out, cells, why = B.narrow_extend(q[:12], q[:12], 5, 24)
+    assert r == 1 and out[0] == 5 + 12 and cells == int(O.extend2_batch(_pairs([(q[:12], q[:12])], 5),
+                                                                          nthreads=1)[0]["cells"][0])
+    # a 60-base exact match from h0 = 5 scores 5 + i after row i: its insertion arm
+    # H(i, i+d) = 5 + i - 6 - d reaches column ~2i while the deletion arm keeps the
+    # band start near i/2, so the band outgrows 24 columns with non-zero cells past it
+    r, out, cells, why = B.narrow_extend(q, q, 5, 24)
+    assert r == 0 and why == 1
+
+
+def _pairs(pairs, h0):
+    from bsbolt_amd.tasks import ExtTasks
+    return ExtTasks.from_pairs(pairs, h0=[h0] * len(pairs), w=100, end_bonus=30)
 
 
 def test_labels_are_smallest_accepted_class(wl_small):
@@ -105,8 +138,8 @@ def test_labels_are_smallest_accepted_class(wl_small):
     s = b.subset(idx)
     lab = B.labels(s)
     for i in range(len(s)):
-        acc = [B.narrow_extend(s.query_of(i), s.target_of(i), int(s.h0[i]), wn, w=int(s.w[i]),
-                               end_bonus=int(s.end_bonus[i]))[0] == 1 for wn in B.WIDTHS]
+        acc = [B.narrow_extend(s.query_of(i), s.target_of(i), int(s.h0[i]), wf, w=int(s.w[i]),
+                               end_bonus=int(s.end_bonus[i]))[0] == 1 for wf in B.FRAMES]
         want = next((c for c in range(4) if acc[c]), 4) if s.qlen[i] <= B.QMAX else 4
         assert lab[i] == want
 

@@ -18,12 +18,9 @@ def _both(wl_small):
     return concat_ext(list(wl_small))
 
 
-def _eligible(b, wn):
-    """Tasks a fixed-class run routes narrow: query <= 256 and clamped band wider than wn."""
-    best = 1
-    lim_i = np.maximum(((b.qlen * best + b.end_bonus - 6) / 1 + 1).astype(np.int64), 1)
-    wcl = np.minimum(b.w, lim_i)
-    return (b.qlen <= 256) & (wcl > wn)
+def _eligible(b):
+    """Tasks a fixed-class run routes to the frame kernel: queries of at most 256 bases."""
+    return b.qlen <= B.QMAX
 
 
 def _policy(engine, policy, cls=0):
@@ -69,14 +66,14 @@ def test_fixed_class_equals_reference(engine, wl_small, cls):
     for k in FIELDS + ("cells",):
         bad = np.nonzero(got[k] != want[k])[0]
         assert len(bad) == 0, f"class {cls}: {k} differs at {len(bad)} tasks, first {bad[:5]}"
-    wn = B.WIDTHS[cls]
-    el = _eligible(b, wn)
+    wf = B.FRAMES[cls]
+    el = _eligible(b)
     assert stats[cls, 0] == int(el.sum())
-    # the GPU certificate rejects exactly where the CPU statement rejects
+    # the GPU frame run falls back exactly where the CPU statement rejects
     idx = np.nonzero(el)[0]
     rej = 0
     for i in idx:
-        r = B.narrow_extend(b.query_of(i), b.target_of(i), int(b.h0[i]), wn, w=int(b.w[i]),
+        r = B.narrow_extend(b.query_of(i), b.target_of(i), int(b.h0[i]), wf, w=int(b.w[i]),
                             end_bonus=int(b.end_bonus[i]))[0]
         rej += r != 1
     assert stats[cls, 1] == rej
@@ -107,7 +104,7 @@ def test_fixed_class_random_scorings(engine, cls):
 @pytest.mark.parametrize("name", ["ext_random_bsbolt.npz", "ext_edge_bsbolt.npz", "ext_brute_bsbolt.npz",
                                   "ext_random_nozdrop.npz", "ext_random_asym.npz", "ext_random_matrix.npz"])
 def test_fixed_class_on_reference_golden(engine, name):
-    """Narrow runs (w = 8 and 32) with their fallback reproduce the reference's
+    """Frame runs (24 and 80 columns) with their fallback reproduce the reference's
     own outputs, including other scoring matrices and gap costs."""
     from bsbolt_amd import bsx
     from conftest import golden_ext
@@ -213,13 +210,13 @@ def test_learned_policy_results_equal_plain(engine, wl_small):
 
 
 def test_trained_predictor_gains(engine):
-    """Train on one simulation, evaluate on another: the trained net routes
-    more tasks to narrow bands that the certificate accepts, and fewer to
-    narrow bands it rejects, than the untrained net; without a predictor no
-    task runs narrow at all."""
+    """Train on one simulation, evaluate on another (same read recipe, disjoint
+    genome and read seeds): the trained net routes more tasks to frames that
+    hold their whole run, and fewer to frames that fall back, than the
+    untrained net."""
     from bsbolt_amd import bsx
-    train = workload_tasks("cfg0", 0.6)
-    test = workload_tasks("cfg3", 0.01)
+    train = workload_tasks("cfg2", 0.03)
+    test = workload_tasks("cfg1", 0.01)
     Xtr, ytr = _feats_labels(engine, train)
     perm = np.random.default_rng(0).permutation(len(train))
     Xtr, ytr = np.ascontiguousarray(Xtr[perm]), np.ascontiguousarray(ytr[perm])

@@ -42,4 +42,4 @@ def test_learned_equals_plain_equals_oracle(engine, cfg):
     narrow = int(st[:4, 0].sum())
     TOTAL[cfg] = len(b)
     print(f"{cfg}: {len(b)} extensions, {narrow} routed narrow, {int(st[:4, 1].sum())} rejected by the "
-          f"certificate, 0 mismatches; running total {sum(TOTAL.values())}")
+          f"frame check, 0 mismatches; running total {sum(TOTAL.values())}")
