@@ -36,8 +36,9 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy")
 TIMED_SLOTS = 16
-# lanes per task and columns per lane of the plain kernel's bins (bsx_internal.h kExtShape)
-EXT_SHAPES = ((16, 1), (16, 2), (16, 3), (16, 4), (16, 6), (16, 8), (32, 6), (32, 8), (64, 8), (64, 16), (64, 32))
+# lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
+# one lane per task is the lane-per-task kernel
+EXT_SHAPES = ((4, 4), (1, 32), (8, 6), (8, 8), (16, 6), (16, 8), (32, 6), (32, 8), (64, 8), (64, 16), (64, 32))
 
 # learned-band form (include/bsx.h)
 BAND_PLAIN, BAND_LEARNED, BAND_FIXED = 0, 1, 2

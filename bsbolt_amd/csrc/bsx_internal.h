@@ -7,13 +7,15 @@ namespace bsx {
 
 // Extension work is split into bins by query length; each bin runs a kernel
 // instantiation with L lanes per task and C query columns per lane
-// (qlen <= L*C).  Index = bin id.
-// Row cost per task ~ (fixed + per-column work * C) * L / 64 wave instructions,
-// so narrow groups (L = 16) with more columns beat wide groups up to C = 8.
+// (qlen <= L*C).  Index = bin id.  L = 1 is the lane-per-task kernel (one task
+// per lane, its whole query in registers, ~22 VALU instructions per column);
+// L >= 16 are the group kernels, whose row cost per task ~ (fixed + per-column
+// work * C) * L / 64 wave instructions carries a fixed part (prefix scans,
+// group reductions) that the lane kernel does not have.
 constexpr int kExtBins = 11;
 struct BinShape { int lanes, cols; };
 constexpr BinShape kExtShape[kExtBins] = {
-    {16, 1}, {16, 2}, {16, 3}, {16, 4}, {16, 6}, {16, 8}, {32, 6}, {32, 8}, {64, 8}, {64, 16}, {64, 32}};
+    {4, 4}, {1, 32}, {8, 6}, {8, 8}, {16, 6}, {16, 8}, {32, 6}, {32, 8}, {64, 8}, {64, 16}, {64, 32}};
 constexpr int kExtBlock = 64;             // threads per extension block: one wave, so a block
                                           // never holds wave slots idle while a sibling finishes
 constexpr int kExtTasksPerBlock = 32;     // task descriptors owned by one plain-kernel block
@@ -105,6 +107,10 @@ inline void time_end(const ExtWork& wk, int slot, hipStream_t st) {
 }
 
 struct ExtKArgs;
+// lane-per-task kernel of bin `bin` (query width Q); grid = resident waves, or
+// fewer when the host knows the bin's task count (hint >= 0)
+template <int Q, bool E1>
+void launch_lane_bin(const ExtKArgs& k, const ExtWork& wk, int bin, int num_cu, hipStream_t st, int64_t hint);
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts, const BandPlan* plan);
 // Tasks per block of the plain and narrow kernels (defaults above; a context

@@ -433,29 +433,36 @@ ext_unpack_kernel(ExtArgs A, ExtWork W, const int32_t* __restrict__ rec) {
 }
 
 template <int B, bool E1>
-static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, hipStream_t st, int64_t hint) {
+static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, int num_cu, hipStream_t st,
+                       int64_t hint) {
   constexpr int L = kExtShape[B].lanes, C = kExtShape[B].cols;
-  // one block per k.tpb descriptors of the bin; without a host-side
-  // count the grid covers the whole batch and surplus blocks exit at once
-  const int64_t n = hint >= 0 ? hint : n_all;
-  const int64_t blocks = (n + k.tpb - 1) / k.tpb;
-  if (blocks < 1) return;
-  time_begin(wk, B, st);
-  hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
-                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
-  time_end(wk, B, st);
+  if constexpr (L == 1) {
+    launch_lane_bin<C, E1>(k, wk, B, num_cu, st, hint);
+  } else {
+    // one block per k.tpb descriptors of the bin; without a host-side
+    // count the grid covers the whole batch and surplus blocks exit at once
+    const int64_t n = hint >= 0 ? hint : n_all;
+    const int64_t blocks = (n + k.tpb - 1) / k.tpb;
+    if (blocks < 1) return;
+    time_begin(wk, B, st);
+    hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
+                       (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
+    time_end(wk, B, st);
+  }
 }
 
 template <int B, bool E1>
-static void launch_bins_from(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st, const int64_t* hc) {
-  launch_bin<B, E1>(k, wk, n, st, hc ? hc[B] : -1);
-  if constexpr (B > 0) launch_bins_from<B - 1, E1>(k, wk, n, st, hc);
+static void launch_bins_from(const ExtKArgs& k, const ExtWork& wk, int64_t n, int num_cu, hipStream_t st,
+                             const int64_t* hc) {
+  launch_bin<B, E1>(k, wk, n, num_cu, st, hc ? hc[B] : -1);
+  if constexpr (B > 0) launch_bins_from<B - 1, E1>(k, wk, n, num_cu, st, hc);
 }
 
 template <bool E1>
-static void launch_bins(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st, const int64_t* hc) {
+static void launch_bins(const ExtKArgs& k, const ExtWork& wk, int64_t n, int num_cu, hipStream_t st,
+                        const int64_t* hc) {
   // longest queries first so the long tasks do not form the tail
-  launch_bins_from<kExtBins - 1, E1>(k, wk, n, st, hc);
+  launch_bins_from<kExtBins - 1, E1>(k, wk, n, num_cu, st, hc);
 }
 
 static int g_tpb_plain = kExtTasksPerBlock, g_tpb_narrow = kNarrowTasksPerBlock;
@@ -498,8 +505,8 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   const int64_t per_block = (a.n + blocks - 1) / blocks;
   hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block,
                      band ? (const uint8_t*)wk.need_plain : nullptr);
-  if (a.e_del == 1 && a.e_ins == 1) launch_bins<true>(k, wk, a.n, st, host_bin_counts);
-  else launch_bins<false>(k, wk, a.n, st, host_bin_counts);
+  if (a.e_del == 1 && a.e_ins == 1) launch_bins<true>(k, wk, a.n, num_cu, st, host_bin_counts);
+  else launch_bins<false>(k, wk, a.n, num_cu, st, host_bin_counts);
   hipLaunchKernelGGL(ext_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, (const int32_t*)wk.rec);
   return hipGetLastError();
 }
