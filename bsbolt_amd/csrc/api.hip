@@ -16,8 +16,8 @@
 
 using namespace bsx;
 
-// counts[kCountSlots], cursor[kCountSlots], err[4], fetch[kExtBins * kExtSlices]
-static constexpr size_t kSmallBytes = 4 * (2 * kCountSlots + 4 + kExtBins * kExtSlices);
+// counts[kCountSlots], cursor[kCountSlots], err[4], fetch[kFetchSlots], split[2 * kLaneFrameClasses]
+static constexpr size_t kSmallBytes = 4 * (2 * kCountSlots + 4 + kFetchSlots + 2 * kLaneFrameClasses);
 
 // band model buffer: params, adam_m, adam_v (FP32), then the BF16 operand copies
 static constexpr size_t kAlign = 256;
@@ -241,6 +241,7 @@ static ExtWork ext_work(bsx_ctx* c) {
   wk.cursor = wk.counts + kCountSlots;
   wk.err = (int32_t*)(wk.cursor + kCountSlots);
   wk.fetch = (uint32_t*)(wk.err + 4);
+  wk.split = wk.fetch + kFetchSlots;
   wk.nbin_of = (uint8_t*)c->ws->nbin_of.p;
   wk.need_plain = (uint8_t*)c->ws->need_plain.p;
   return wk;

@@ -26,7 +26,7 @@ constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; 
 constexpr int kGlbBins = 8;
 constexpr BinShape kGlbShape[kGlbBins] = {
     {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
-constexpr int kExtSlices = 8;  // fetch-counter slots per CIGAR bin
+constexpr int kExtSlices = 8;  // fetch-counter slots per bin (CIGAR bins, lane-kernel bins and classes)
 constexpr int glb_bin_limit(int b) { return kGlbShape[b].lanes * kGlbShape[b].cols; }
 
 // Learned-band form: classes 0..3 run the band-following frame kernel with
@@ -36,6 +36,12 @@ constexpr int kBandClasses = 5;
 constexpr int kCountSlots = 16;
 static_assert(kExtBins + kBandClasses - 1 <= kCountSlots, "count slots for bins and narrow classes");
 constexpr int kNarrowQMax = 256;  // longest query the narrow kernels take
+// Frame classes run by the lane-per-task frame kernel (one task per lane) for
+// queries of at most kLaneFrameQMax bases; longer queries of these classes run
+// the group frame kernel.  Both compute the same frame run.
+constexpr int kLaneFrameClasses = 1;
+constexpr int kLaneFrameQMax = 128;
+constexpr int kFetchSlots = kCountSlots * kExtSlices;
 enum BandMode { kBandPlain = 0, kBandLearned = 1, kBandFixed = 2 };
 struct BandPlan {
   int mode;            // BandMode
@@ -84,7 +90,9 @@ struct ExtWork {
   int32_t* rec;        // [n][8] packed per-task results
   uint32_t* counts;    // [kCountSlots] tasks per bin / narrow class
   uint32_t* cursor;    // [kCountSlots] scatter cursors
-  uint32_t* fetch;     // [kExtBins * kExtSlices] persistent-kernel fetch counters
+  uint32_t* fetch;     // [kFetchSlots] persistent-kernel fetch counters, kExtSlices per count slot
+  uint32_t* split;     // [2 * kLaneFrameClasses]: per lane-frame class, tasks at the front of its
+                       // list (query <= kLaneFrameQMax) and the back-of-list cursor
   int32_t* err;        // [0] rejected count, [1] lowest rejected index
   uint8_t* nbin_of;    // [n] narrow class of the task, 0xff = plain only (learned-band form)
   uint8_t* need_plain; // [n] 1 = the plain kernel computes the task (learned-band form)
@@ -111,6 +119,9 @@ struct ExtKArgs;
 // fewer when the host knows the bin's task count (hint >= 0)
 template <int Q, bool E1>
 void launch_lane_bin(const ExtKArgs& k, const ExtWork& wk, int bin, int num_cu, hipStream_t st, int64_t hint);
+// lane-per-task frame kernel (frame of WF columns) of lane-frame class `cls`
+template <int WF, bool E1>
+void launch_lane_frame(const ExtKArgs& k, const ExtWork& wk, int cls, int num_cu, hipStream_t st);
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts, const BandPlan* plan);
 // Tasks per block of the plain and narrow kernels (defaults above; a context

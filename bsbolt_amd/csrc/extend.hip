@@ -410,7 +410,8 @@ ext_bin_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, const uint8_t* _
 __global__ void ext_reset_kernel(ExtWork W) {
   const int k = threadIdx.x;
   if (k < kCountSlots) { W.counts[k] = 0; W.cursor[k] = 0; }
-  if (k < kExtBins * kExtSlices) W.fetch[k] = 0;
+  if (k < kFetchSlots) W.fetch[k] = 0;
+  if (k < 2 * kLaneFrameClasses) W.split[k] = 0;
   if (k == 0) { W.err[0] = 0; W.err[1] = 0x7fffffff; }
 }
 
@@ -474,7 +475,7 @@ void set_tasks_per_block(int plain, int narrow) {
 
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts, const BandPlan* plan) {
-  hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(kExtBins * kExtSlices), 0, st, wk);
+  hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(kFetchSlots), 0, st, wk);
   if (a.n <= 0) return hipGetLastError();
   int64_t blocks = (a.n + 255) / 256;
   if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;

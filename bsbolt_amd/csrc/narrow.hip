@@ -35,16 +35,17 @@ __host__ __device__ constexpr int frame_lanes(int c) { return c <= 1 ? 8 : 16; }
 __host__ __device__ constexpr int frame_cols(int c) { return c == 0 ? 3 : c == 1 ? 5 : c == 2 ? 5 : 9; }
 __host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) * frame_cols(c); }
 
-// Estimated VALU issue cost of one DP row per task (wave instructions / tasks
-// per wave): a fixed part per row plus a part per column held by each lane.
-// A task goes to a narrow class only where that class is cheaper than the
-// plain kernel's bin for its query length.
-__host__ __device__ constexpr int row_cost(int lanes, int cols, int fixed) {
-  return (fixed + 45 * cols) * lanes / 64;
-}
-__host__ __device__ constexpr bool narrow_pays(int c, int b) {  // >= 20% cheaper
-  return 10 * row_cost(frame_lanes(c), frame_cols(c), 140) <
-         8 * row_cost(kExtShape[b].lanes, kExtShape[b].cols, 110);
+// Measured cost of one DP row per task on MI355X (picoseconds of kernel time
+// per row over a 150 bp paired-end bisulfite batch): the plain form's bins
+// (4-lane, lane-per-task, 8-, 16-, 32- and 64-lane kernels) and the frame
+// classes (lane-per-task 24-column frame, group frames of 40, 80 and 144
+// columns).  A task goes to its predicted frame class only where that class is
+// at least 20% cheaper per row than the plain bin of its query length.
+constexpr int kPlainRowPs[kExtBins] = {52, 53, 76, 94, 120, 166, 590, 700, 1000, 2000, 4000};
+constexpr int kFrameRowPs[kBandClasses - 1] = {27, 59, 152, 239};
+static_assert(kExtBins == 11 && kBandClasses == 5, "row cost tables");
+__host__ __device__ constexpr bool narrow_pays(int c, int b) {
+  return 10 * kFrameRowPs[c] < 8 * kPlainRowPs[b];
 }
 
 __device__ __forceinline__ int ramp_at(int j, int h0, int oe_ins, int e_ins) {  // eh[j].h before row 0
@@ -76,7 +77,7 @@ __device__ __forceinline__ int grp_or(int x, int lane) {
 template <int L, int C, bool E1>
 __global__ void __launch_bounds__(kExtBlock)
 ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t* __restrict__ counts,
-                  int cls, uint8_t* __restrict__ need_plain) {
+                  const uint32_t* __restrict__ split, int cls, uint8_t* __restrict__ need_plain) {
   constexpr int WF = L * C;
   constexpr int NG = kExtBlock / L;
   constexpr int QD = (kNarrowQMax + 6) / 4;  // aligned dwords that can cover a query
@@ -96,7 +97,9 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
   uint32_t lbase = 0;
   for (int b = 0; b < cls; ++b) lbase += counts[kExtBins + b];
   const uint32_t nlist = counts[kExtBins + cls];
-  const uint32_t blk0 = blockIdx.x * (uint32_t)A.tpb;
+  // the short-query tasks that lead a lane-frame class's list run in the lane kernel
+  const uint32_t first = cls < kLaneFrameClasses ? split[cls] : 0u;
+  const uint32_t blk0 = first + blockIdx.x * (uint32_t)A.tpb;
   if (blk0 >= nlist) return;  // uniform over the block
   const uint32_t blk1 = min(blk0 + (uint32_t)A.tpb, nlist);
   if (threadIdx.x == 0) lds_next = blk0;
@@ -392,8 +395,9 @@ __device__ __forceinline__ bool narrow_pays_dev(int c, int b) {
 // kernel's for its query length.
 __global__ void __launch_bounds__(256)
 ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
-  __shared__ uint32_t hist[kBandClasses];
+  __shared__ uint32_t hist[kBandClasses], front[kLaneFrameClasses];
   if (threadIdx.x < kBandClasses) hist[threadIdx.x] = 0;
+  if (threadIdx.x < kLaneFrameClasses) front[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -406,23 +410,33 @@ ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
     W.nbin_of[t] = (uint8_t)nb;
     W.need_plain[t] = nb == 0xff ? 1 : 0;
     if (nb != 0xff) atomicAdd(&hist[nb], 1u);
+    if (nb < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax) atomicAdd(&front[nb], 1u);
   }
   __syncthreads();
   if (threadIdx.x < kBandClasses && hist[threadIdx.x])
     atomicAdd(&W.counts[kExtBins + threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x < kLaneFrameClasses && front[threadIdx.x]) atomicAdd(&W.split[threadIdx.x], front[threadIdx.x]);
 }
 
 __global__ void __launch_bounds__(256)
 ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
+  // A lane-frame class's list holds its short-query tasks at the front (front
+  // cursor, counting up) and the others at the back (back cursor, counting
+  // down from the end); the other classes fill their lists from the front.
   __shared__ uint32_t off[kBandClasses], hist[kBandClasses], cur[kBandClasses];
+  __shared__ uint32_t boff[kLaneFrameClasses], bhist[kLaneFrameClasses], bcur[kLaneFrameClasses];
   const int64_t t0 = (int64_t)blockIdx.x * per_block;
   const int64_t t1 = min(A.n, t0 + per_block);
   if (threadIdx.x < kBandClasses) { hist[threadIdx.x] = 0; cur[threadIdx.x] = 0; }
+  if (threadIdx.x < kLaneFrameClasses) { bhist[threadIdx.x] = 0; bcur[threadIdx.x] = 0; }
   __syncthreads();
+  auto back_of = [&](int64_t t, int nb) { return nb < kLaneFrameClasses && A.qlen[t] > kLaneFrameQMax; };
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     if (W.bin_of[t] == 0xff) continue;
     const int nb = W.nbin_of[t];
-    if (nb != 0xff) atomicAdd(&hist[nb], 1u);
+    if (nb == 0xff) continue;
+    if (back_of(t, nb)) atomicAdd(&bhist[nb], 1u);
+    else atomicAdd(&hist[nb], 1u);
   }
   __syncthreads();
   if (threadIdx.x < kBandClasses) {
@@ -430,45 +444,52 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
     for (int b = 0; b < (int)threadIdx.x; ++b) s += W.counts[kExtBins + b];
     const uint32_t hc = hist[threadIdx.x];
     off[threadIdx.x] = s + (hc ? atomicAdd(&W.cursor[kExtBins + threadIdx.x], hc) : 0u);
+    if (threadIdx.x < kLaneFrameClasses) {  // last slot of the class's list minus the back cursor
+      const uint32_t bc = bhist[threadIdx.x];
+      boff[threadIdx.x] = s + W.counts[kExtBins + threadIdx.x] - 1u -
+                          (bc ? atomicAdd(&W.split[kLaneFrameClasses + threadIdx.x], bc) : 0u);
+    }
   }
   __syncthreads();
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     if (W.bin_of[t] == 0xff) continue;
     const int nb = W.nbin_of[t];
     if (nb == 0xff) continue;
-    const uint32_t p = off[nb] + atomicAdd(&cur[nb], 1u);
+    const uint32_t p = back_of(t, nb) ? boff[nb] - atomicAdd(&bcur[nb], 1u) : off[nb] + atomicAdd(&cur[nb], 1u);
     write_desc(A, t, W.desc + p);
   }
 }
 
 template <int CL, bool E1>
-static void launch_narrow_class(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st) {
+static void launch_narrow_class(const ExtKArgs& k, const ExtWork& wk, int64_t n, int num_cu, hipStream_t st) {
   constexpr int L = frame_lanes(CL), C = frame_cols(CL);
   const int64_t blocks = (n + k.tpb - 1) / k.tpb;
   if (blocks < 1) return;
   time_begin(wk, kExtBins + CL, st);
+  if constexpr (CL < kLaneFrameClasses) launch_lane_frame<frame_of_class(CL), E1>(k, wk, CL, num_cu, st);
   hipLaunchKernelGGL((ext_narrow_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
-                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, CL, wk.need_plain);
+                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, (const uint32_t*)wk.split, CL,
+                     wk.need_plain);
   time_end(wk, kExtBins + CL, st);
 }
 
 template <bool E1>
-static void launch_narrow_classes(const ExtKArgs& k, const ExtWork& wk, int64_t n, hipStream_t st,
+static void launch_narrow_classes(const ExtKArgs& k, const ExtWork& wk, int64_t n, int num_cu, hipStream_t st,
                                   const BandPlan& plan) {
   if (plan.mode == kBandFixed) {  // one class only
     switch (plan.fixed) {
-      case 0: launch_narrow_class<0, E1>(k, wk, n, st); break;
-      case 1: launch_narrow_class<1, E1>(k, wk, n, st); break;
-      case 2: launch_narrow_class<2, E1>(k, wk, n, st); break;
-      case 3: launch_narrow_class<3, E1>(k, wk, n, st); break;
+      case 0: launch_narrow_class<0, E1>(k, wk, n, num_cu, st); break;
+      case 1: launch_narrow_class<1, E1>(k, wk, n, num_cu, st); break;
+      case 2: launch_narrow_class<2, E1>(k, wk, n, num_cu, st); break;
+      case 3: launch_narrow_class<3, E1>(k, wk, n, num_cu, st); break;
       default: break;
     }
     return;
   }
-  launch_narrow_class<3, E1>(k, wk, n, st);
-  launch_narrow_class<2, E1>(k, wk, n, st);
-  launch_narrow_class<1, E1>(k, wk, n, st);
-  launch_narrow_class<0, E1>(k, wk, n, st);
+  launch_narrow_class<3, E1>(k, wk, n, num_cu, st);
+  launch_narrow_class<2, E1>(k, wk, n, num_cu, st);
+  launch_narrow_class<1, E1>(k, wk, n, num_cu, st);
+  launch_narrow_class<0, E1>(k, wk, n, num_cu, st);
 }
 
 hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st,
@@ -478,8 +499,8 @@ hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk,
   hipLaunchKernelGGL(ext_narrow_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, plan);
   const int64_t per_block = (a.n + blocks - 1) / blocks;
   hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
-  if (a.e_del == 1 && a.e_ins == 1) launch_narrow_classes<true>(k, wk, a.n, st, plan);
-  else launch_narrow_classes<false>(k, wk, a.n, st, plan);
+  if (a.e_del == 1 && a.e_ins == 1) launch_narrow_classes<true>(k, wk, a.n, num_cu, st, plan);
+  else launch_narrow_classes<false>(k, wk, a.n, num_cu, st, plan);
   return hipGetLastError();
 }
 
