@@ -208,7 +208,8 @@ def main():
     bin_of = np.searchsorted(limits, batch.qlen)
     per_launch = kms[:len(limits)] / max(kcalls, 1)
     dom = int(np.argmax(per_launch))
-    dom_name = f"ext_plain_kernel<{bsx.EXT_SHAPES[dom][0]}, {bsx.EXT_SHAPES[dom][1]}, true>"
+    lanes, cols = bsx.EXT_SHAPES[dom]
+    dom_name = f"ext_lane_kernel<{cols}, true>" if lanes == 1 else f"ext_plain_kernel<{lanes}, {cols}, true>"
     dom_cells = int(cells[bin_of == dom].sum())
     dom_ms = float(per_launch[dom])
     dom_tops = dom_cells * OPS_PER_CELL / (dom_ms / 1e3) / 1e12

@@ -31,7 +31,9 @@
 
 namespace bsx {
 
-__host__ __device__ constexpr int frame_lanes(int c) { return c <= 1 ? 8 : 16; }  // frames 24, 40, 80, 144
+// group shapes of the frames 24, 40, 80, 144 (lanes x columns per lane; fewer
+// lanes with more columns each measured slower for the 40- and 80-column frames)
+__host__ __device__ constexpr int frame_lanes(int c) { return c <= 1 ? 8 : 16; }
 __host__ __device__ constexpr int frame_cols(int c) { return c == 0 ? 3 : c == 1 ? 5 : c == 2 ? 5 : 9; }
 __host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) * frame_cols(c); }
 
@@ -59,6 +61,7 @@ __device__ __forceinline__ int grp_or(int x, int lane) {
   if (L <= 16) {
     x |= __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true);
     x |= __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true);
+    if (L == 4) return x;
     x |= __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true);
     if (L == 16) x |= __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true);
     return x;
