@@ -38,7 +38,7 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
-EXT_SHAPES = ((4, 4), (1, 32), (8, 6), (8, 8), (16, 6), (16, 8), (32, 6), (32, 8), (64, 8), (64, 16), (64, 32))
+EXT_SHAPES = ((4, 4), (1, 32), (1, 48), (1, 64), (16, 6), (16, 8), (32, 6), (32, 8), (64, 8), (64, 16), (64, 32))
 
 # learned-band form (include/bsx.h)
 BAND_PLAIN, BAND_LEARNED, BAND_FIXED = 0, 1, 2

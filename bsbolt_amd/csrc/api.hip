@@ -68,6 +68,7 @@ struct bsx_ctx {
   int num_cu = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
+  StreamSet ss;  // side streams of the extension launches
   std::string err;
   Work wown;           // workspace of the eager calls
   Work* ws = &wown;    // workspace in use (a graph's while it is captured)
@@ -163,6 +164,20 @@ bsx_ctx* bsx_create(int device) {
   c->num_cu = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
   c->stream = c->own;
+  {
+    bool ok = hipEventCreateWithFlags(&c->ss.fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < StreamSet::kSide; ++k)
+      ok = hipStreamCreateWithFlags(&c->ss.side[k], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&c->ss.join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {  // run everything on the call's stream
+      for (int k = 0; k < StreamSet::kSide; ++k) {
+        if (c->ss.side[k]) (void)hipStreamDestroy(c->ss.side[k]);
+        if (c->ss.join[k]) (void)hipEventDestroy(c->ss.join[k]);
+      }
+      if (c->ss.fork) (void)hipEventDestroy(c->ss.fork);
+      c->ss = StreamSet{};
+    }
+  }
   if (hipHostMalloc((void**)&c->err_host, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
     (void)hipStreamDestroy(c->own);
     delete c;
@@ -190,6 +205,14 @@ void bsx_destroy(bsx_ctx* c) {
   if (c->err_host) (void)hipHostFree(c->err_host);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (int k = 0; k < StreamSet::kSide; ++k) {
+    if (c->ss.side[k]) {
+      (void)hipStreamSynchronize(c->ss.side[k]);
+      (void)hipStreamDestroy(c->ss.side[k]);
+    }
+    if (c->ss.join[k]) (void)hipEventDestroy(c->ss.join[k]);
+  }
+  if (c->ss.fork) (void)hipEventDestroy(c->ss.fork);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -244,6 +267,7 @@ static ExtWork ext_work(bsx_ctx* c) {
   wk.split = wk.fetch + kFetchSlots;
   wk.nbin_of = (uint8_t*)c->ws->nbin_of.p;
   wk.need_plain = (uint8_t*)c->ws->need_plain.p;
+  wk.ss = &c->ss;
   return wk;
 }
 

@@ -15,7 +15,7 @@ namespace bsx {
 constexpr int kExtBins = 11;
 struct BinShape { int lanes, cols; };
 constexpr BinShape kExtShape[kExtBins] = {
-    {4, 4}, {1, 32}, {8, 6}, {8, 8}, {16, 6}, {16, 8}, {32, 6}, {32, 8}, {64, 8}, {64, 16}, {64, 32}};
+    {4, 4}, {1, 32}, {1, 48}, {1, 64}, {16, 6}, {16, 8}, {32, 6}, {32, 8}, {64, 8}, {64, 16}, {64, 32}};
 constexpr int kExtBlock = 64;             // threads per extension block: one wave, so a block
                                           // never holds wave slots idle while a sibling finishes
 constexpr int kExtTasksPerBlock = 32;     // task descriptors owned by one plain-kernel block
@@ -39,7 +39,7 @@ constexpr int kNarrowQMax = 256;  // longest query the narrow kernels take
 // Frame classes run by the lane-per-task frame kernel (one task per lane) for
 // queries of at most kLaneFrameQMax bases; longer queries of these classes run
 // the group frame kernel.  Both compute the same frame run.
-constexpr int kLaneFrameClasses = 1;
+constexpr int kLaneFrameClasses = 2;
 constexpr int kLaneFrameQMax = 128;
 constexpr int kFetchSlots = kCountSlots * kExtSlices;
 enum BandMode { kBandPlain = 0, kBandLearned = 1, kBandFixed = 2 };
@@ -83,6 +83,32 @@ struct ExtDesc {
 };
 static_assert(sizeof(ExtDesc) == 48, "descriptor layout");
 
+// Side streams of a context: independent launches (the plain bins, the frame
+// classes) are spread over the call's stream and these, between a fork and a
+// join, so that one launch's tail overlaps the others' work.  All streams keep
+// the default priority.
+struct StreamSet {
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr, join[kSide] = {};
+};
+// launches of the plain bins: stream 0 = the call's stream, k > 0 = side[k-1]
+constexpr int kBinStream[11] = {2, 1, 3, 1, 0, 2, 3, 0, 0, 0, 0};
+inline hipStream_t pick_stream(const StreamSet* ss, hipStream_t main, int k) {
+  return (ss && k > 0 && ss->side[k - 1]) ? ss->side[k - 1] : main;
+}
+inline void ss_fork(const StreamSet* ss, hipStream_t main) {
+  if (!ss || !ss->fork) return;
+  (void)hipEventRecord(ss->fork, main);
+  for (int k = 0; k < StreamSet::kSide; ++k) (void)hipStreamWaitEvent(ss->side[k], ss->fork, 0);
+}
+inline void ss_join(const StreamSet* ss, hipStream_t main) {
+  if (!ss || !ss->fork) return;
+  for (int k = 0; k < StreamSet::kSide; ++k) {
+    (void)hipEventRecord(ss->join[k], ss->side[k]);
+    (void)hipStreamWaitEvent(main, ss->join[k], 0);
+  }
+}
 // Per-call device workspace for the extension path.
 struct ExtWork {
   uint8_t* bin_of;     // [n] bin id, 0xff = rejected task
@@ -101,6 +127,7 @@ struct ExtWork {
   // gets bit `slot` set for every pair recorded
   hipEvent_t* tev = nullptr;
   uint32_t* tmask = nullptr;
+  const StreamSet* ss = nullptr;  // side streams (nullable: everything on the call's stream)
 };
 constexpr int kTimedSlots = 16;  // kExtBins + 4 narrow classes + predictor
 static_assert(kExtBins + kBandClasses <= kTimedSlots, "timing slots");

@@ -455,7 +455,7 @@ static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, int 
 template <int B, bool E1>
 static void launch_bins_from(const ExtKArgs& k, const ExtWork& wk, int64_t n, int num_cu, hipStream_t st,
                              const int64_t* hc) {
-  launch_bin<B, E1>(k, wk, n, num_cu, st, hc ? hc[B] : -1);
+  launch_bin<B, E1>(k, wk, n, num_cu, pick_stream(wk.ss, st, kBinStream[B]), hc ? hc[B] : -1);
   if constexpr (B > 0) launch_bins_from<B - 1, E1>(k, wk, n, num_cu, st, hc);
 }
 
@@ -506,8 +506,10 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   const int64_t per_block = (a.n + blocks - 1) / blocks;
   hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block,
                      band ? (const uint8_t*)wk.need_plain : nullptr);
+  ss_fork(wk.ss, st);
   if (a.e_del == 1 && a.e_ins == 1) launch_bins<true>(k, wk, a.n, num_cu, st, host_bin_counts);
   else launch_bins<false>(k, wk, a.n, num_cu, st, host_bin_counts);
+  ss_join(wk.ss, st);
   hipLaunchKernelGGL(ext_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, (const int32_t*)wk.rec);
   return hipGetLastError();
 }

@@ -300,8 +300,11 @@ ext_lane_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
           f = max(max(M - oe_ins, f - e_ins), 0);
           nz[c >> 5] |= (uint32_t)min(G[c] | E[c], 1) << (c & 31);
           le = lt;
-          // keep the columns in order: hoisting the work of all Q columns to the
-          // row start would hold Q lane masks and temporaries live at once
+          // Keep the columns in order and each column's results materialised at
+          // its end: otherwise the compiler hoists the compares of all Q columns
+          // to the row start and sinks the state updates to their last use,
+          // holding Q lane masks and temporaries live at once.
+          asm volatile("" ::"v"(G[c]), "v"(E[c]), "v"(h1), "v"(f), "v"(mk));
           __builtin_amdgcn_sched_barrier(0);
         }
         // only columns j < end count in the zero scans
@@ -652,6 +655,7 @@ ext_lane_frame_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const ui
           nz[c >> 5] |= (uint32_t)min(G[c] | E[c], 1) << (c & 31);
           if (c == WF - 1) hlast = h;
           le = lt;
+          asm volatile("" ::"v"(G[c]), "v"(E[c]), "v"(h1), "v"(f), "v"(mk));  // as in the lane kernel
           __builtin_amdgcn_sched_barrier(0);
         }
         fail = fail || (spill && hlast != 0);  // a non-zero cell past the frame
@@ -769,11 +773,15 @@ void launch_lane_frame(const ExtKArgs& k, const ExtWork& wk, int cls, int num_cu
 }
 template void launch_lane_frame<24, true>(const ExtKArgs&, const ExtWork&, int, int, hipStream_t);
 template void launch_lane_frame<24, false>(const ExtKArgs&, const ExtWork&, int, int, hipStream_t);
+template void launch_lane_frame<40, true>(const ExtKArgs&, const ExtWork&, int, int, hipStream_t);
+template void launch_lane_frame<40, false>(const ExtKArgs&, const ExtWork&, int, int, hipStream_t);
 
 #define BSX_LANE_INST(Q)                                                                                         \
   template void launch_lane_bin<Q, true>(const ExtKArgs&, const ExtWork&, int, int, hipStream_t, int64_t);     \
   template void launch_lane_bin<Q, false>(const ExtKArgs&, const ExtWork&, int, int, hipStream_t, int64_t);
 BSX_LANE_INST(32)
+BSX_LANE_INST(48)
+BSX_LANE_INST(64)
 #undef BSX_LANE_INST
 
 }  // namespace bsx

@@ -56,49 +56,69 @@ __device__ __forceinline__ bf4 ld_bf4(const uint16_t* p) {  // 8-byte aligned
 __device__ void band_features(const ExtArgs& A, int64_t t, float* f) {
   const int qlen = A.qlen[t], tlen = A.tlen[t], h0 = A.h0[t], eb = A.end_bonus[t];
   const int oe_ins = A.o_ins + A.e_ins, oe_del = A.o_del + A.e_del;
-  const uint8_t* q = A.query + A.q_off[t];
-  const uint8_t* tg = A.target + A.t_off[t];
+  static_assert(kFeatScan == 16, "the scan reads 16 query and 19 target bases");
+  // The bases the scans read, with all loads issued up front: aligned dwords
+  // (indices clamped to the sequence's last dword, so only mapped dwords that
+  // hold sequence bases are read), realigned to byte 0.  A task without target
+  // bases reads its query instead and masks every target position.
+  uint32_t qb[4], tb[5];
+  {
+    const uintptr_t qa = (uintptr_t)(A.query + A.q_off[t]);
+    const uint32_t* qp = (const uint32_t*)(qa & ~(uintptr_t)3);
+    const int qsh = (int)(qa & 3), qlast = (qsh + (qlen > 0 ? qlen : 1) - 1) >> 2;
+    uint32_t w[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k] = qp[min(k, qlast)];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) qb[k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], (uint32_t)qsh);
+    const uintptr_t ta = (uintptr_t)(tlen > 0 ? A.target + A.t_off[t] : A.query + A.q_off[t]);
+    const uint32_t* tp = (const uint32_t*)(ta & ~(uintptr_t)3);
+    const int tsh = (int)(ta & 3), tlast = (tsh + (tlen > 0 ? tlen : 1) - 1) >> 2;
+    uint32_t v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = tp[min(k, tlast)];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) tb[k] = __builtin_amdgcn_alignbyte(v[k + 1], v[k], (uint32_t)tsh);
+  }
+  auto qbase = [&](int p) { return (int)((qb[p >> 2] >> (8 * (p & 3))) & 0xffu); };
+  auto tbase = [&](int r) { return (r >= 0 && r < tlen) ? (int)((tb[r >> 2] >> (8 * (r & 3))) & 0xffu) : 5; };
   const int K = qlen < kFeatScan ? qlen : kFeatScan;
   int S[5] = {0, 0, 0, 0, 0}, MX[5] = {0, 0, 0, 0, 0};
   int arg0 = 0, match0 = 0, mism0 = 0, first_mm = K, match16 = 0;
   int qT = 0, qN = 0, gc = 0, cpg = 0;
-  int tw[5];  // target bases p-2 .. p+2 (code 5 = outside the window)
 #pragma unroll
-  for (int d = 0; d < 5; ++d) {
-    const int r = d - 2;
-    tw[d] = (r >= 0 && r < tlen) ? tg[r] : 5;
-  }
-  for (int p = 0; p < K; ++p) {
-    const uint32_t qc = q[p];
-    qT += qc == 3;
-    qN += qc > 3;
-    uint32_t lo = A.prof_lo[4];  // mat[k*5 + qc], k = 0..3
-    int hi = A.prof_hi[4];       // mat[20 + qc]
+  for (int p = 0; p < kFeatScan; ++p) {
+    if (p < K) {
+      const uint32_t qc = (uint32_t)qbase(p);
+      qT += qc == 3;
+      qN += qc > 3;
+      uint32_t lo = A.prof_lo[4];  // mat[k*5 + qc], k = 0..3
+      int hi = A.prof_hi[4];       // mat[20 + qc]
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      lo = qc == (uint32_t)k ? A.prof_lo[k] : lo;
-      hi = qc == (uint32_t)k ? A.prof_hi[k] : hi;
+      for (int k = 0; k < 4; ++k) {
+        lo = qc == (uint32_t)k ? A.prof_lo[k] : lo;
+        hi = qc == (uint32_t)k ? A.prof_hi[k] : hi;
+      }
+      int tw[5];  // target bases p-2 .. p+2 (5 = outside the target)
+#pragma unroll
+      for (int d = 0; d < 5; ++d) tw[d] = tbase(p + d - 2);
+#pragma unroll
+      for (int d = 0; d < 5; ++d) {
+        if (tw[d] == 5) continue;
+        S[d] += tw[d] < 4 ? sbyte(lo, tw[d]) : hi;
+        MX[d] = S[d] > MX[d] ? S[d] : MX[d];
+      }
+      if (tw[2] != 5) {
+        const bool eq = (int)qc == tw[2] && qc < 4;
+        match0 += eq;
+        mism0 += !eq;
+        match16 += (eq && p < 16);
+        if (!eq && first_mm == K) first_mm = p;
+        if (S[2] == MX[2] && S[2] > 0) arg0 = p + 1;
+        gc += tw[2] == 1 || tw[2] == 2;
+        cpg += tw[2] == 1 && tw[3] == 2;
+      }
     }
-#pragma unroll
-    for (int d = 0; d < 5; ++d) {
-      if (tw[d] == 5) continue;
-      S[d] += tw[d] < 4 ? sbyte(lo, tw[d]) : hi;
-      MX[d] = S[d] > MX[d] ? S[d] : MX[d];
-    }
-    if (tw[2] != 5) {
-      const bool eq = (int)qc == tw[2] && qc < 4;
-      match0 += eq;
-      mism0 += !eq;
-      match16 += (eq && p < 16);
-      if (!eq && first_mm == K) first_mm = p;
-      if (S[2] == MX[2] && S[2] > 0) arg0 = p + 1;
-      gc += tw[2] == 1 || tw[2] == 2;
-      cpg += tw[2] == 1 && tw[3] == 2;
-    }
-#pragma unroll
-    for (int d = 0; d < 4; ++d) tw[d] = tw[d + 1];
-    const int r = p + 3;
-    tw[4] = r < tlen ? tg[r] : 5;
   }
   int wcl = A.w[t];
   {

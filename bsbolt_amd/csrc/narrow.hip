@@ -38,13 +38,14 @@ __host__ __device__ constexpr int frame_cols(int c) { return c == 0 ? 3 : c == 1
 __host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) * frame_cols(c); }
 
 // Measured cost of one DP row per task on MI355X (picoseconds of kernel time
-// per row over a 150 bp paired-end bisulfite batch): the plain form's bins
-// (4-lane, lane-per-task, 8-, 16-, 32- and 64-lane kernels) and the frame
-// classes (lane-per-task 24-column frame, group frames of 40, 80 and 144
-// columns).  A task goes to its predicted frame class only where that class is
-// at least 20% cheaper per row than the plain bin of its query length.
+// per row over a 150 bp paired-end bisulfite batch, each kernel run alone):
+// the plain form's bins and the frame classes (lane-per-task frames of 24 and
+// 40 columns -- upper bounds, from the rows of the runs that completed --
+// and group frames of 80 and 144 columns).  A task goes to its predicted frame
+// class only where that class is at least 20% cheaper per row than the plain
+// bin of its query length.
 constexpr int kPlainRowPs[kExtBins] = {52, 53, 76, 94, 120, 166, 590, 700, 1000, 2000, 4000};
-constexpr int kFrameRowPs[kBandClasses - 1] = {27, 59, 152, 239};
+constexpr int kFrameRowPs[kBandClasses - 1] = {40, 55, 152, 239};
 static_assert(kExtBins == 11 && kBandClasses == 5, "row cost tables");
 __host__ __device__ constexpr bool narrow_pays(int c, int b) {
   return 10 * kFrameRowPs[c] < 8 * kPlainRowPs[b];
@@ -468,12 +469,14 @@ static void launch_narrow_class(const ExtKArgs& k, const ExtWork& wk, int64_t n,
   constexpr int L = frame_lanes(CL), C = frame_cols(CL);
   const int64_t blocks = (n + k.tpb - 1) / k.tpb;
   if (blocks < 1) return;
-  time_begin(wk, kExtBins + CL, st);
-  if constexpr (CL < kLaneFrameClasses) launch_lane_frame<frame_of_class(CL), E1>(k, wk, CL, num_cu, st);
-  hipLaunchKernelGGL((ext_narrow_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
+  // classes on separate streams; a lane-frame class's lane kernel beside its group kernel
+  const hipStream_t sg = pick_stream(wk.ss, st, CL == 3 ? 0 : CL + 1);
+  time_begin(wk, kExtBins + CL, sg);
+  if constexpr (CL < kLaneFrameClasses) launch_lane_frame<frame_of_class(CL), E1>(k, wk, CL, num_cu, sg);
+  hipLaunchKernelGGL((ext_narrow_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, sg, k,
                      (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, (const uint32_t*)wk.split, CL,
                      wk.need_plain);
-  time_end(wk, kExtBins + CL, st);
+  time_end(wk, kExtBins + CL, sg);
 }
 
 template <bool E1>
@@ -502,8 +505,10 @@ hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk,
   hipLaunchKernelGGL(ext_narrow_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, plan);
   const int64_t per_block = (a.n + blocks - 1) / blocks;
   hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
+  ss_fork(wk.ss, st);
   if (a.e_del == 1 && a.e_ins == 1) launch_narrow_classes<true>(k, wk, a.n, num_cu, st, plan);
   else launch_narrow_classes<false>(k, wk, a.n, num_cu, st, plan);
+  ss_join(wk.ss, st);
   return hipGetLastError();
 }
 
