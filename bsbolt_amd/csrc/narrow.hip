@@ -423,7 +423,7 @@ ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
 }
 
 __global__ void __launch_bounds__(256)
-ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
+ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, int cascade) {
   // A lane-frame class's list holds its short-query tasks at the front (front
   // cursor, counting up) and the others at the back (back cursor, counting
   // down from the end); the other classes fill their lists from the front.
@@ -435,10 +435,20 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
   if (threadIdx.x < kLaneFrameClasses) { bhist[threadIdx.x] = 0; bcur[threadIdx.x] = 0; }
   __syncthreads();
   auto back_of = [&](int64_t t, int nb) { return nb < kLaneFrameClasses && A.qlen[t] > kLaneFrameQMax; };
+  // in a cascade pass only the tasks the cascade count marked (bit 7) take part
+  auto member = [&](int64_t t, int& nb) {
+    if (W.bin_of[t] == 0xff) return false;
+    nb = W.nbin_of[t];
+    if (nb == 0xff) return false;
+    if (cascade) {
+      if (!(nb & 0x80)) return false;
+      nb &= 0x7f;
+    }
+    return true;
+  };
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-    if (W.bin_of[t] == 0xff) continue;
-    const int nb = W.nbin_of[t];
-    if (nb == 0xff) continue;
+    int nb;
+    if (!member(t, nb)) continue;
     if (back_of(t, nb)) atomicAdd(&bhist[nb], 1u);
     else atomicAdd(&hist[nb], 1u);
   }
@@ -456,11 +466,41 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
   }
   __syncthreads();
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-    if (W.bin_of[t] == 0xff) continue;
-    const int nb = W.nbin_of[t];
-    if (nb == 0xff) continue;
+    int nb;
+    if (!member(t, nb)) continue;
     const uint32_t p = back_of(t, nb) ? boff[nb] - atomicAdd(&bcur[nb], 1u) : off[nb] + atomicAdd(&cur[nb], 1u);
     write_desc(A, t, W.desc + p);
+    if (cascade) W.nbin_of[t] = (uint8_t)nb;
+  }
+}
+
+// Cascade pass: a task whose frame run of class `from` fell back is routed to
+// class `to` (its plain-kernel flag cleared again); the class lists and their
+// counters are rebuilt for this pass only.
+__global__ void ext_cascade_reset_kernel(ExtWork W) {
+  const int k = threadIdx.x;
+  if (k < kBandClasses) { W.counts[kExtBins + k] = 0; W.cursor[kExtBins + k] = 0; }
+  if (k < 2 * kLaneFrameClasses) W.split[k] = 0;
+  if (k < kBandClasses * kExtSlices) W.fetch[kExtBins * kExtSlices + k] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+ext_cascade_count_kernel(ExtArgs A, ExtWork W, int from, int to) {
+  __shared__ uint32_t n_to, n_front;
+  if (threadIdx.x == 0) { n_to = 0; n_front = 0; }
+  __syncthreads();
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (W.bin_of[t] == 0xff || W.nbin_of[t] != from || !W.need_plain[t]) continue;
+    W.nbin_of[t] = (uint8_t)(to | 0x80);
+    W.need_plain[t] = 0;
+    atomicAdd(&n_to, 1u);
+    if (to < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax) atomicAdd(&n_front, 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (n_to) atomicAdd(&W.counts[kExtBins + to], n_to);
+    if (n_front && to < kLaneFrameClasses) atomicAdd(&W.split[to], n_front);
   }
 }
 
@@ -504,10 +544,24 @@ hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk,
   if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
   hipLaunchKernelGGL(ext_narrow_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, plan);
   const int64_t per_block = (a.n + blocks - 1) / blocks;
-  hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
+  hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block, 0);
   ss_fork(wk.ss, st);
   if (a.e_del == 1 && a.e_ins == 1) launch_narrow_classes<true>(k, wk, a.n, num_cu, st, plan);
   else launch_narrow_classes<false>(k, wk, a.n, num_cu, st, plan);
+  ss_join(wk.ss, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_narrow_cascade(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st) {
+  int64_t blocks = (a.n + 255) / 256;
+  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  hipLaunchKernelGGL(ext_cascade_reset_kernel, dim3(1), dim3(kBandClasses * kExtSlices), 0, st, wk);
+  hipLaunchKernelGGL(ext_cascade_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, 0, 1);
+  const int64_t per_block = (a.n + blocks - 1) / blocks;
+  hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block, 1);
+  ss_fork(wk.ss, st);  // the class runs on a side stream, after the routing above
+  if (a.e_del == 1 && a.e_ins == 1) launch_narrow_class<1, true>(k, wk, a.n, num_cu, st);
+  else launch_narrow_class<1, false>(k, wk, a.n, num_cu, st);
   ss_join(wk.ss, st);
   return hipGetLastError();
 }
