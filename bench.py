@@ -86,11 +86,15 @@ def learned_leg(args, eng, ptrs, outs, cells_d, plain_out, plain_cells, n, total
             eng.sync()
             mism = int(sum(int((outs[k].numpy() != plain_out[k]).sum()) for k in RESULT_FIELDS))
             mism += int((cells_d.numpy() != plain_cells).sum())
-            eng.set_kernel_timing(args.steps)
             ms = time_steps(eng, ptrs, args.steps)
+            st = eng.band_stats()
+            # kernel breakdown with the launches in order (each kernel alone on the GPU)
+            eng.set_side_streams(False)
+            eng.set_kernel_timing(args.steps)
+            ms_in_order = time_steps(eng, ptrs, args.steps)
             kms, kcalls = eng.kernel_times()
             eng.set_kernel_timing(0)
-            st = eng.band_stats()
+            eng.set_side_streams(True)
         finally:
             eng.set_band_policy(bsx.BAND_PLAIN)
         routed = {w: int(st[c, 0]) for c, w in enumerate(("f24", "f40", "f80", "f144", "plain"))}
@@ -101,6 +105,7 @@ def learned_leg(args, eng, ptrs, outs, cells_d, plain_out, plain_cells, n, total
                  "routed": routed, "rejected": rejected,
                  "narrow_accepted": narrow - int(st[:4, 1].sum()),
                  "fallback_rate": (int(st[:4, 1].sum()) / narrow) if narrow else 0.0,
+                 "ms_per_step_launches_in_order": ms_in_order,
                  "kernel_ms_per_step": {"predict": float(kms[15] / max(kcalls, 1)),
                                         "narrow": float(kms[11:15].sum() / max(kcalls, 1)),
                                         "plain": float(kms[:11].sum() / max(kcalls, 1))}}
@@ -183,9 +188,8 @@ def main():
     nominal = int((batch.qlen.astype(np.int64) * batch.tlen).sum())
 
     # timed region: K steps bracketed by stream synchronisation; HIP events on
-    # the stream the kernels run on
+    # the context's stream, which joins the side streams at the end of a step
     eng.sync()
-    eng.set_kernel_timing(args.steps)  # HIP events around every kernel of the timed steps
     w0 = time.perf_counter()
     eng.timer_start()
     for _ in range(args.steps):
@@ -193,8 +197,18 @@ def main():
     ms = eng.timer_stop()
     eng.sync()
     wall = time.perf_counter() - w0
+    # per-kernel times for the roofline: the same steps with the launches in
+    # order, so that each kernel has the GPU to itself (HIP events around every
+    # kernel, on the stream it is launched on)
+    eng.set_side_streams(False)
+    eng.set_kernel_timing(args.steps)
+    eng.timer_start()
+    for _ in range(args.steps):
+        eng.extend_device(ptrs)
+    ms_in_order = eng.timer_stop() / args.steps
     kms, kcalls = eng.kernel_times()
     eng.set_kernel_timing(0)
+    eng.set_side_streams(True)
     ms_step = ms / args.steps
     ext_s = n / (ms_step / 1e3)
     gcups = total_cells / (ms_step / 1e3) / 1e9
@@ -251,7 +265,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": dom_name, "achieved": dom_tops, "peak": PEAK_PK16_TOPS,
                      "unit": "TOP/s", "frac": dom_tops / PEAK_PK16_TOPS, "frac_int32": dom_tops / PEAK_INT32_TOPS,
                      "traffic": traffic, "ops_per_launch": dom_cells * OPS_PER_CELL, "ops_per_cell": OPS_PER_CELL,
-                     "ms_per_launch": dom_ms, "share_of_step": dom_ms / ms_step,
+                     "ms_per_launch": dom_ms, "share_of_step_in_order": dom_ms / ms_in_order,
+                     "ms_per_step_launches_in_order": ms_in_order,
                      "algorithmic_bytes_per_launch": dom_bytes,
                      "kernel_ms_per_step": {f"bin<={int(l)}": float(t) for l, t in zip(limits, per_launch) if t > 0}},
         "roofline_step": {"bound": "valu", "achieved": tops, "peak": PEAK_PK16_TOPS, "unit": "TOP/s",

@@ -34,7 +34,8 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get", "bsx_band_features_device",
            "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
-           "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy")
+           "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
+           "bsx_set_side_streams")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -148,7 +149,8 @@ def lib():
         L.bsx_set_kernel_timing.argtypes = [P, C.c_int]
         L.bsx_kernel_times.argtypes = [P, P, C.c_int, P]
         L.bsx_ext_bin_limits.argtypes = [P, C.c_int]
-        for f in ("bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits"):
+        L.bsx_set_side_streams.argtypes = [P, C.c_int]
+        for f in ("bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits", "bsx_set_side_streams"):
             getattr(L, f).restype = C.c_int
         for f in ("bsx_set_band_policy", "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get",
                   "bsx_band_features_device", "bsx_band_labels_device", "bsx_band_forward_device",
@@ -340,6 +342,10 @@ class Engine:
         return score, ncig, cig
 
     # ---- per-kernel timing ----
+    def set_side_streams(self, enable):
+        """Spread independent launches over side streams (default) or run them in order."""
+        self._check(self.L.bsx_set_side_streams(self.ctx, 1 if enable else 0))
+
     def set_kernel_timing(self, max_calls):
         self._check(self.L.bsx_set_kernel_timing(self.ctx, int(max_calls)))
 

@@ -69,6 +69,7 @@ struct bsx_ctx {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   StreamSet ss;  // side streams of the extension launches
+  bool ss_on = true;
   std::string err;
   Work wown;           // workspace of the eager calls
   Work* ws = &wown;    // workspace in use (a graph's while it is captured)
@@ -267,7 +268,7 @@ static ExtWork ext_work(bsx_ctx* c) {
   wk.split = wk.fetch + kFetchSlots;
   wk.nbin_of = (uint8_t*)c->ws->nbin_of.p;
   wk.need_plain = (uint8_t*)c->ws->need_plain.p;
-  wk.ss = &c->ss;
+  wk.ss = c->ss_on ? &c->ss : nullptr;
   return wk;
 }
 
@@ -336,6 +337,12 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   c->last_n = plan.mode == kBandPlain ? 0 : (int64_t)n;
   c->pending_check = true;
   c->pending_kind = 1;
+  return BSX_OK;
+}
+
+int bsx_set_side_streams(bsx_ctx* c, int enable) {
+  if (!c || (enable != 0 && enable != 1)) return fail(c, BSX_EINVAL, "enable must be 0 or 1");
+  c->ss_on = enable != 0;
   return BSX_OK;
 }
 

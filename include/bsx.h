@@ -220,6 +220,13 @@ void bsx_graph_destroy(bsx_graph *graph);
 int bsx_set_kernel_timing(bsx_ctx *ctx, int max_calls);
 int bsx_kernel_times(bsx_ctx *ctx, float *ms, int n_slots, int *calls);
 int bsx_ext_bin_limits(int32_t *limits, int n);
+/* bsx_set_side_streams(ctx, 0|1): the extension calls spread their independent
+ * launches (query-length bins, frame classes) over the context's stream and
+ * three side streams of its own (1, the default), or run them one after the
+ * other on the context's stream (0: each kernel then has the GPU to itself,
+ * which is how per-kernel times are taken for the roofline).  No counterpart
+ * in the reference. */
+int bsx_set_side_streams(bsx_ctx *ctx, int enable);
 
 /* Scalar drop-in with the exact signature of ksw_extend2 (ksw.h:108), run as
  * a one-task batch on the calling thread's default context (device 0).  m must
