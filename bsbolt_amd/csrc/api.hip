@@ -16,8 +16,11 @@
 
 using namespace bsx;
 
-// counts[kCountSlots], cursor[kCountSlots], err[4], fetch[kFetchSlots], split[2 * kLaneFrameClasses]
-static constexpr size_t kSmallBytes = 4 * (2 * kCountSlots + 4 + kFetchSlots + 2 * kLaneFrameClasses);
+// counts[kCountSlots], cursor[kCountSlots], err[4], fetch[kFetchSlots], split[2 * kLaneFrameClasses],
+// bcount[kExtBins * kRowBuckets], bcursor[kExtBins * kRowBuckets],
+// fcount[kLaneFrameClasses * kRowBuckets], fcursor[kLaneFrameClasses * kRowBuckets]
+static constexpr size_t kSmallBytes = 4 * (2 * kCountSlots + 4 + kFetchSlots + 2 * kLaneFrameClasses +
+                                           2 * kExtBins * kRowBuckets + 2 * kLaneFrameClasses * kRowBuckets);
 
 // band model buffer: params, adam_m, adam_v (FP32), then the BF16 operand copies
 static constexpr size_t kAlign = 256;
@@ -266,6 +269,10 @@ static ExtWork ext_work(bsx_ctx* c) {
   wk.err = (int32_t*)(wk.cursor + kCountSlots);
   wk.fetch = (uint32_t*)(wk.err + 4);
   wk.split = wk.fetch + kFetchSlots;
+  wk.bcount = wk.split + 2 * kLaneFrameClasses;
+  wk.bcursor = wk.bcount + kExtBins * kRowBuckets;
+  wk.fcount = wk.bcursor + kExtBins * kRowBuckets;
+  wk.fcursor = wk.fcount + kLaneFrameClasses * kRowBuckets;
   wk.nbin_of = (uint8_t*)c->ws->nbin_of.p;
   wk.need_plain = (uint8_t*)c->ws->need_plain.p;
   wk.ss = c->ss_on ? &c->ss : nullptr;

@@ -28,6 +28,33 @@ constexpr BinShape kGlbShape[kGlbBins] = {
     {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
 constexpr int kExtSlices = 8;  // fetch-counter slots per bin (CIGAR bins, lane-kernel bins and classes)
 constexpr int glb_bin_limit(int b) { return kGlbShape[b].lanes * kGlbShape[b].cols; }
+// First entry of fetch slice s of a list of n entries: a balanced partition,
+// the first n % kExtSlices slices hold one entry more.
+__host__ __device__ inline uint32_t slice_begin(uint32_t n, int s) {
+  const uint32_t q = n / kExtSlices, r = n % kExtSlices;
+  return (uint32_t)s * q + ((uint32_t)s < r ? (uint32_t)s : r);
+}
+
+// Plain bins list their tasks longest run first, so that a kernel's last
+// tasks are short ones and its waves drain together.  The rows a run takes
+// follow min(tlen, 2*h0) closely (the run ends when the target does, or once
+// the score built on h0 is spent or the z-drop fires); the key is bucketed in
+// kRowBuckets steps of 32 rows, bucket 0 = longest.  Lane-kernel bins place
+// rank r of that order at slot r / kExtSlices of fetch slice r % kExtSlices,
+// so that every slice is itself longest first.  The order changes no result.
+constexpr int kRowBuckets = 8;
+__host__ __device__ inline int row_bucket(int tlen, int h0) {
+  const int est = tlen < 2 * h0 ? tlen : 2 * h0;
+  const int k = est >> 5;
+  return kRowBuckets - 1 - (k < kRowBuckets - 1 ? (k > 0 ? k : 0) : kRowBuckets - 1);
+}
+constexpr uint32_t lane_bin_mask() {
+  uint32_t m = 0;
+  for (int b = 0; b < kExtBins; ++b)
+    if (kExtShape[b].lanes == 1) m |= 1u << b;
+  return m;
+}
+constexpr uint32_t kLaneBinMask = lane_bin_mask();
 
 // Learned-band form: classes 0..3 run the band-following frame kernel with
 // frames of 24, 40, 80, 144 columns; class 4 is the plain kernel.  Count slots: plain bins
@@ -123,6 +150,10 @@ struct ExtWork {
   int32_t* err;        // [0] rejected count, [1] lowest rejected index
   uint8_t* nbin_of;    // [n] narrow class of the task, 0xff = plain only (learned-band form)
   uint8_t* need_plain; // [n] 1 = the plain kernel computes the task (learned-band form)
+  uint32_t* bcount;    // [kExtBins * kRowBuckets] tasks per plain bin and row bucket
+  uint32_t* bcursor;   // [kExtBins * kRowBuckets] scatter cursors of the same
+  uint32_t* fcount;    // [kLaneFrameClasses * kRowBuckets] front tasks per lane-frame class and row bucket
+  uint32_t* fcursor;   // [kLaneFrameClasses * kRowBuckets] scatter cursors of the same
   // optional per-kernel timing: events [2*slot], [2*slot+1] around each kernel
   // launch (slots: plain bins, then narrow classes, then the predictor); *tmask
   // gets bit `slot` set for every pair recorded

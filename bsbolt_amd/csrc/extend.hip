@@ -340,20 +340,25 @@ __device__ __forceinline__ int ext_bin_of(int qlen) {
   return -1;
 }
 
-// mode 0: validate every task, write bin_of and count per bin
+// mode 0: validate every task, write bin_of and count per bin and row bucket
 // mode 1: validate only (the learned-band form counts after its narrow pass)
 // mode 2: count the valid tasks still flagged in need[] (no validation)
 __global__ void __launch_bounds__(256)
 ext_bin_count_kernel(ExtArgs A, ExtWork W, int mode) {
-  __shared__ uint32_t hist[kExtBins];
+  constexpr int NB = kExtBins * kRowBuckets;
+  __shared__ uint32_t hist[kExtBins], bhist[NB];
   if (threadIdx.x < kExtBins) hist[threadIdx.x] = 0;
+  if (threadIdx.x < NB) bhist[threadIdx.x] = 0;
   __syncthreads();
   int bad = 0, first_bad = 0x7fffffff;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
     if (mode == 2) {
       const int b = W.bin_of[t];
-      if (b != 0xff && W.need_plain[t]) atomicAdd(&hist[b], 1u);
+      if (b != 0xff && W.need_plain[t]) {
+        atomicAdd(&hist[b], 1u);
+        atomicAdd(&bhist[b * kRowBuckets + row_bucket(A.tlen[t], A.h0[t])], 1u);
+      }
       continue;
     }
     const int ql = A.qlen[t], tl = A.tlen[t], h0 = A.h0[t], w = A.w[t], eb = A.end_bonus[t];
@@ -366,6 +371,7 @@ ext_bin_count_kernel(ExtArgs A, ExtWork W, int mode) {
       first_bad = min(first_bad, (int)t);
     } else if (mode == 0) {
       atomicAdd(&hist[b], 1u);
+      atomicAdd(&bhist[b * kRowBuckets + row_bucket(tl, h0)], 1u);
     }
     W.bin_of[t] = (uint8_t)b;
   }
@@ -375,43 +381,59 @@ ext_bin_count_kernel(ExtArgs A, ExtWork W, int mode) {
   }
   __syncthreads();
   if (threadIdx.x < kExtBins && hist[threadIdx.x]) atomicAdd(&W.counts[threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x < NB && bhist[threadIdx.x]) atomicAdd(&W.bcount[threadIdx.x], bhist[threadIdx.x]);
 }
 
-// Each block handles a contiguous range of tasks: it counts them per bin in
-// LDS, reserves one range per bin with a single global atomic, and writes the
-// descriptors of its tasks into those ranges.
+// Each block handles a contiguous range of tasks: it counts them per bin and
+// row bucket in LDS, reserves one range per (bin, bucket) with a single global
+// atomic, and writes the descriptors of its tasks into those ranges.  A task's
+// rank in its bin's longest-first order is its bucket's base plus its place in
+// the bucket; lane-kernel bins spread the ranks over the fetch slices.
 __global__ void __launch_bounds__(256)
 ext_bin_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, const uint8_t* __restrict__ need) {
-  __shared__ uint32_t off[kExtBins], hist[kExtBins], cur[kExtBins];
+  constexpr int NB = kExtBins * kRowBuckets;
+  __shared__ uint32_t off[NB], hist[NB], cur[NB], lbase[kExtBins], lsize[kExtBins];
   const int64_t t0 = (int64_t)blockIdx.x * per_block;
   const int64_t t1 = min(A.n, t0 + per_block);
-  if (threadIdx.x < kExtBins) { hist[threadIdx.x] = 0; cur[threadIdx.x] = 0; }
+  if (threadIdx.x < NB) { hist[threadIdx.x] = 0; cur[threadIdx.x] = 0; }
   __syncthreads();
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int b = W.bin_of[t];
-    if (b != 0xff && (!need || need[t])) atomicAdd(&hist[b], 1u);
+    if (b != 0xff && (!need || need[t])) atomicAdd(&hist[b * kRowBuckets + row_bucket(A.tlen[t], A.h0[t])], 1u);
   }
   __syncthreads();
+  if (threadIdx.x < NB) {
+    const int b = threadIdx.x / kRowBuckets, k = threadIdx.x % kRowBuckets;
+    uint32_t s = 0;
+    for (int j = 0; j < k; ++j) s += W.bcount[b * kRowBuckets + j];
+    const uint32_t hcount = hist[threadIdx.x];
+    off[threadIdx.x] = s + (hcount ? atomicAdd(&W.bcursor[threadIdx.x], hcount) : 0u);
+  }
   if (threadIdx.x < kExtBins) {
     uint32_t s = 0;
     for (int b = 0; b < (int)threadIdx.x; ++b) s += W.counts[b];
-    const uint32_t hcount = hist[threadIdx.x];
-    off[threadIdx.x] = s + (hcount ? atomicAdd(&W.cursor[threadIdx.x], hcount) : 0u);
+    lbase[threadIdx.x] = s;
+    lsize[threadIdx.x] = W.counts[threadIdx.x];
   }
   __syncthreads();
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int b = W.bin_of[t];
     if (b == 0xff || (need && !need[t])) continue;
-    const uint32_t p = off[b] + atomicAdd(&cur[b], 1u);
-    write_desc(A, t, W.desc + p);
+    const int kb = b * kRowBuckets + row_bucket(A.tlen[t], A.h0[t]);
+    const uint32_t r = off[kb] + atomicAdd(&cur[kb], 1u);
+    const uint32_t p = (kLaneBinMask >> b) & 1u ? slice_begin(lsize[b], (int)(r % kExtSlices)) + r / kExtSlices : r;
+    write_desc(A, t, W.desc + lbase[b] + p);
   }
 }
 
+static_assert(kExtBins * kRowBuckets <= kFetchSlots && kExtBins * kRowBuckets <= 256, "bucket counters");
 __global__ void ext_reset_kernel(ExtWork W) {
   const int k = threadIdx.x;
   if (k < kCountSlots) { W.counts[k] = 0; W.cursor[k] = 0; }
   if (k < kFetchSlots) W.fetch[k] = 0;
   if (k < 2 * kLaneFrameClasses) W.split[k] = 0;
+  if (k < kExtBins * kRowBuckets) { W.bcount[k] = 0; W.bcursor[k] = 0; }
+  if (k < kLaneFrameClasses * kRowBuckets) { W.fcount[k] = 0; W.fcursor[k] = 0; }
   if (k == 0) { W.err[0] = 0; W.err[1] = 0x7fffffff; }
 }
 
@@ -477,8 +499,11 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
                          const int64_t* host_bin_counts, const BandPlan* plan) {
   hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(kFetchSlots), 0, st, wk);
   if (a.n <= 0) return hipGetLastError();
-  int64_t blocks = (a.n + 255) / 256;
+  int64_t blocks = (a.n + 255) / 256;  // unpack pass
   if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  // count and scatter passes: each block ends with one global atomic per
+  // non-empty bin and row bucket, so fewer, larger blocks
+  const int64_t sblocks = blocks < (int64_t)num_cu * 4 ? blocks : (int64_t)num_cu * 4;
   const bool band = plan && plan->mode != kBandPlain;
   ExtKArgs k{};
   k.query = a.query; k.target = a.target; k.rec = wk.rec;
@@ -494,18 +519,18 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
     k.trow_lo[t] = lo;
     k.trow_hi[t] = t < 4 ? (a.prof_lo[4] >> (8 * t)) & 0xffu : (uint32_t)a.prof_hi[4] & 0xffu;
   }
-  hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, band ? 1 : 0);
+  hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)sblocks), dim3(256), 0, st, a, wk, band ? 1 : 0);
   if (band) {
     ExtKArgs kn = k;
     kn.tpb = g_tpb_narrow;
     hipError_t e = launch_narrow(a, kn, wk, num_cu, st, *plan);
     if (e == hipSuccess && plan->cascade) e = launch_narrow_cascade(a, kn, wk, num_cu, st);
     if (e != hipSuccess || plan->skip_plain) return e;
-    hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, 2);
+    hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)sblocks), dim3(256), 0, st, a, wk, 2);
     host_bin_counts = nullptr;
   }
-  const int64_t per_block = (a.n + blocks - 1) / blocks;
-  hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block,
+  const int64_t per_block = (a.n + sblocks - 1) / sblocks;
+  hipLaunchKernelGGL(ext_bin_scatter_kernel, dim3((unsigned)sblocks), dim3(256), 0, st, a, wk, per_block,
                      band ? (const uint8_t*)wk.need_plain : nullptr);
   ss_fork(wk.ss, st);
   if (a.e_del == 1 && a.e_ins == 1) launch_bins<true>(k, wk, a.n, num_cu, st, host_bin_counts);

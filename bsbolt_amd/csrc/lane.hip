@@ -199,8 +199,8 @@ ext_lane_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
     if (apend) {
       const uint32_t old = __builtin_amdgcn_readfirstlane(aret);
       apend = false;
-      const uint32_t s0 = (uint32_t)((uint64_t)nlist * acs / kExtSlices);
-      const uint32_t s1 = (uint32_t)((uint64_t)nlist * (acs + 1) / kExtSlices);
+      const uint32_t s0 = slice_begin(nlist, acs);
+      const uint32_t s1 = slice_begin(nlist, acs + 1);
       const uint32_t size = s1 - s0;
       if (old < size) {
         const uint32_t lo = s0 + old, hi = s0 + (old + CH < size ? old + CH : size);
@@ -539,8 +539,8 @@ ext_lane_frame_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const ui
     if (apend) {
       const uint32_t old = __builtin_amdgcn_readfirstlane(aret);
       apend = false;
-      const uint32_t s0 = (uint32_t)((uint64_t)nlist * acs / kExtSlices);
-      const uint32_t s1 = (uint32_t)((uint64_t)nlist * (acs + 1) / kExtSlices);
+      const uint32_t s0 = slice_begin(nlist, acs);
+      const uint32_t s1 = slice_begin(nlist, acs + 1);
       const uint32_t size = s1 - s0;
       if (old < size) {
         const uint32_t lo = s0 + old, hi = s0 + (old + CH < size ? old + CH : size);

@@ -399,9 +399,11 @@ __device__ __forceinline__ bool narrow_pays_dev(int c, int b) {
 // kernel's for its query length.
 __global__ void __launch_bounds__(256)
 ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
-  __shared__ uint32_t hist[kBandClasses], front[kLaneFrameClasses];
+  constexpr int NF = kLaneFrameClasses * kRowBuckets;
+  __shared__ uint32_t hist[kBandClasses], front[kLaneFrameClasses], fhist[NF];
   if (threadIdx.x < kBandClasses) hist[threadIdx.x] = 0;
   if (threadIdx.x < kLaneFrameClasses) front[threadIdx.x] = 0;
+  if (threadIdx.x < NF) fhist[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -414,26 +416,36 @@ ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
     W.nbin_of[t] = (uint8_t)nb;
     W.need_plain[t] = nb == 0xff ? 1 : 0;
     if (nb != 0xff) atomicAdd(&hist[nb], 1u);
-    if (nb < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax) atomicAdd(&front[nb], 1u);
+    if (nb < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax) {
+      atomicAdd(&front[nb], 1u);
+      atomicAdd(&fhist[nb * kRowBuckets + row_bucket(A.tlen[t], A.h0[t])], 1u);
+    }
   }
   __syncthreads();
   if (threadIdx.x < kBandClasses && hist[threadIdx.x])
     atomicAdd(&W.counts[kExtBins + threadIdx.x], hist[threadIdx.x]);
   if (threadIdx.x < kLaneFrameClasses && front[threadIdx.x]) atomicAdd(&W.split[threadIdx.x], front[threadIdx.x]);
+  if (threadIdx.x < NF && fhist[threadIdx.x]) atomicAdd(&W.fcount[threadIdx.x], fhist[threadIdx.x]);
 }
 
 __global__ void __launch_bounds__(256)
 ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, int cascade) {
-  // A lane-frame class's list holds its short-query tasks at the front (front
-  // cursor, counting up) and the others at the back (back cursor, counting
-  // down from the end); the other classes fill their lists from the front.
+  // A lane-frame class's list holds its short-query tasks at the front and
+  // the others at the back (back cursor, counting down from the end); the
+  // other classes fill their lists from the front.  The front is the lane
+  // frame kernel's list: longest run first, in row buckets, each rank placed
+  // in its fetch slice as in the lane-kernel bins.
+  constexpr int NF = kLaneFrameClasses * kRowBuckets;
   __shared__ uint32_t off[kBandClasses], hist[kBandClasses], cur[kBandClasses];
   __shared__ uint32_t boff[kLaneFrameClasses], bhist[kLaneFrameClasses], bcur[kLaneFrameClasses];
+  __shared__ uint32_t foff[NF], fhist[NF], fcur[NF], fbase[kLaneFrameClasses], fsize[kLaneFrameClasses];
   const int64_t t0 = (int64_t)blockIdx.x * per_block;
   const int64_t t1 = min(A.n, t0 + per_block);
   if (threadIdx.x < kBandClasses) { hist[threadIdx.x] = 0; cur[threadIdx.x] = 0; }
   if (threadIdx.x < kLaneFrameClasses) { bhist[threadIdx.x] = 0; bcur[threadIdx.x] = 0; }
+  if (threadIdx.x < NF) { fhist[threadIdx.x] = 0; fcur[threadIdx.x] = 0; }
   __syncthreads();
+  auto front_of = [&](int64_t t, int nb) { return nb < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax; };
   auto back_of = [&](int64_t t, int nb) { return nb < kLaneFrameClasses && A.qlen[t] > kLaneFrameQMax; };
   // in a cascade pass only the tasks the cascade count marked (bit 7) take part
   auto member = [&](int64_t t, int& nb) {
@@ -450,9 +462,17 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, int cascade) 
     int nb;
     if (!member(t, nb)) continue;
     if (back_of(t, nb)) atomicAdd(&bhist[nb], 1u);
+    else if (front_of(t, nb)) atomicAdd(&fhist[nb * kRowBuckets + row_bucket(A.tlen[t], A.h0[t])], 1u);
     else atomicAdd(&hist[nb], 1u);
   }
   __syncthreads();
+  if (threadIdx.x < NF) {
+    const int c = threadIdx.x / kRowBuckets, k = threadIdx.x % kRowBuckets;
+    uint32_t s = 0;
+    for (int j = 0; j < k; ++j) s += W.fcount[c * kRowBuckets + j];
+    const uint32_t hc = fhist[threadIdx.x];
+    foff[threadIdx.x] = s + (hc ? atomicAdd(&W.fcursor[threadIdx.x], hc) : 0u);
+  }
   if (threadIdx.x < kBandClasses) {
     uint32_t s = 0;
     for (int b = 0; b < (int)threadIdx.x; ++b) s += W.counts[kExtBins + b];
@@ -462,13 +482,24 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, int cascade) 
       const uint32_t bc = bhist[threadIdx.x];
       boff[threadIdx.x] = s + W.counts[kExtBins + threadIdx.x] - 1u -
                           (bc ? atomicAdd(&W.split[kLaneFrameClasses + threadIdx.x], bc) : 0u);
+      fbase[threadIdx.x] = s;
+      fsize[threadIdx.x] = W.split[threadIdx.x];
     }
   }
   __syncthreads();
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     int nb;
     if (!member(t, nb)) continue;
-    const uint32_t p = back_of(t, nb) ? boff[nb] - atomicAdd(&bcur[nb], 1u) : off[nb] + atomicAdd(&cur[nb], 1u);
+    uint32_t p;
+    if (back_of(t, nb)) {
+      p = boff[nb] - atomicAdd(&bcur[nb], 1u);
+    } else if (front_of(t, nb)) {
+      const int kb = nb * kRowBuckets + row_bucket(A.tlen[t], A.h0[t]);
+      const uint32_t r = foff[kb] + atomicAdd(&fcur[kb], 1u);
+      p = fbase[nb] + slice_begin(fsize[nb], (int)(r % kExtSlices)) + r / kExtSlices;
+    } else {
+      p = off[nb] + atomicAdd(&cur[nb], 1u);
+    }
     write_desc(A, t, W.desc + p);
     if (cascade) W.nbin_of[t] = (uint8_t)nb;
   }
@@ -482,12 +513,15 @@ __global__ void ext_cascade_reset_kernel(ExtWork W) {
   if (k < kBandClasses) { W.counts[kExtBins + k] = 0; W.cursor[kExtBins + k] = 0; }
   if (k < 2 * kLaneFrameClasses) W.split[k] = 0;
   if (k < kBandClasses * kExtSlices) W.fetch[kExtBins * kExtSlices + k] = 0;
+  if (k < kLaneFrameClasses * kRowBuckets) { W.fcount[k] = 0; W.fcursor[k] = 0; }
 }
+static_assert(kLaneFrameClasses * kRowBuckets <= kBandClasses * kExtSlices, "cascade reset threads");
 
 __global__ void __launch_bounds__(256)
 ext_cascade_count_kernel(ExtArgs A, ExtWork W, int from, int to) {
-  __shared__ uint32_t n_to, n_front;
+  __shared__ uint32_t n_to, n_front, fhist[kRowBuckets];
   if (threadIdx.x == 0) { n_to = 0; n_front = 0; }
+  if (threadIdx.x < kRowBuckets) fhist[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -495,13 +529,18 @@ ext_cascade_count_kernel(ExtArgs A, ExtWork W, int from, int to) {
     W.nbin_of[t] = (uint8_t)(to | 0x80);
     W.need_plain[t] = 0;
     atomicAdd(&n_to, 1u);
-    if (to < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax) atomicAdd(&n_front, 1u);
+    if (to < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax) {
+      atomicAdd(&n_front, 1u);
+      atomicAdd(&fhist[row_bucket(A.tlen[t], A.h0[t])], 1u);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     if (n_to) atomicAdd(&W.counts[kExtBins + to], n_to);
     if (n_front && to < kLaneFrameClasses) atomicAdd(&W.split[to], n_front);
   }
+  if (to < kLaneFrameClasses && threadIdx.x < kRowBuckets && fhist[threadIdx.x])
+    atomicAdd(&W.fcount[to * kRowBuckets + threadIdx.x], fhist[threadIdx.x]);
 }
 
 template <int CL, bool E1>
@@ -540,8 +579,9 @@ static void launch_narrow_classes(const ExtKArgs& k, const ExtWork& wk, int64_t 
 
 hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st,
                          const BandPlan& plan) {
+  // count and scatter: each block ends with global atomics per class and row bucket
   int64_t blocks = (a.n + 255) / 256;
-  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  if (blocks > (int64_t)num_cu * 4) blocks = (int64_t)num_cu * 4;
   hipLaunchKernelGGL(ext_narrow_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, plan);
   const int64_t per_block = (a.n + blocks - 1) / blocks;
   hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block, 0);
@@ -554,7 +594,7 @@ hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk,
 
 hipError_t launch_narrow_cascade(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st) {
   int64_t blocks = (a.n + 255) / 256;
-  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  if (blocks > (int64_t)num_cu * 4) blocks = (int64_t)num_cu * 4;
   hipLaunchKernelGGL(ext_cascade_reset_kernel, dim3(1), dim3(kBandClasses * kExtSlices), 0, st, wk);
   hipLaunchKernelGGL(ext_cascade_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, 0, 1);
   const int64_t per_block = (a.n + blocks - 1) / blocks;
