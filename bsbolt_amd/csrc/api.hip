@@ -334,7 +334,7 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
     time_begin(wk, kTimedSlots - 1, c->stream);
     HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), (uint8_t*)c->ws->cls.p, c->num_cu, c->stream), "band predict");
     time_end(wk, kTimedSlots - 1, c->stream);
-    plan = BandPlan{kBandLearned, 0, (const uint8_t*)c->ws->cls.p, false, false, true};
+    plan = BandPlan{kBandLearned, 0, (const uint8_t*)c->ws->cls.p, false, false};
   } else if (c->band_policy == BSX_BAND_FIXED) {
     plan = BandPlan{kBandFixed, c->band_class, nullptr, true, false};
   }

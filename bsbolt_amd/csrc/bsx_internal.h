@@ -76,7 +76,6 @@ struct BandPlan {
   const uint8_t* cls;  // [n] predicted class for kBandLearned
   bool any_gain;       // route a task to its class even where the frame is not narrower than its bin
   bool skip_plain;     // narrow pass only (labelling): no plain pass, no result unpack
-  bool cascade = false;  // tasks whose 24-column frame run fell back retry the 40-column frame
 };
 
 // Scores are tracked as (h << 11 | column) keys in 32-bit lanes.
@@ -186,7 +185,6 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
 // Tasks per block of the plain and narrow kernels (defaults above; a context
 // may override them, e.g. from BSX_TPB_PLAIN / BSX_TPB_NARROW).
 void set_tasks_per_block(int plain, int narrow);
-hipError_t launch_narrow_cascade(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st);
 hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st,
                          const BandPlan& plan);
 

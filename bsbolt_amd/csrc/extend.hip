@@ -524,7 +524,6 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
     ExtKArgs kn = k;
     kn.tpb = g_tpb_narrow;
     hipError_t e = launch_narrow(a, kn, wk, num_cu, st, *plan);
-    if (e == hipSuccess && plan->cascade) e = launch_narrow_cascade(a, kn, wk, num_cu, st);
     if (e != hipSuccess || plan->skip_plain) return e;
     hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)sblocks), dim3(256), 0, st, a, wk, 2);
     host_bin_counts = nullptr;

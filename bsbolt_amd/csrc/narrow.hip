@@ -44,8 +44,8 @@ __host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) 
 // and group frames of 80 and 144 columns).  A task goes to its predicted frame
 // class only where that class is at least 20% cheaper per row than the plain
 // bin of its query length.
-constexpr int kPlainRowPs[kExtBins] = {52, 53, 76, 94, 120, 166, 590, 700, 1000, 2000, 4000};
-constexpr int kFrameRowPs[kBandClasses - 1] = {40, 55, 152, 239};
+constexpr int kPlainRowPs[kExtBins] = {42, 43, 57, 59, 109, 158, 600, 700, 1000, 2000, 4000};
+constexpr int kFrameRowPs[kBandClasses - 1] = {29, 45, 152, 239};
 static_assert(kExtBins == 11 && kBandClasses == 5, "row cost tables");
 __host__ __device__ constexpr bool narrow_pays(int c, int b) {
   return 10 * kFrameRowPs[c] < 8 * kPlainRowPs[b];
@@ -429,7 +429,7 @@ ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
 }
 
 __global__ void __launch_bounds__(256)
-ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, int cascade) {
+ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block) {
   // A lane-frame class's list holds its short-query tasks at the front and
   // the others at the back (back cursor, counting down from the end); the
   // other classes fill their lists from the front.  The front is the lane
@@ -447,16 +447,10 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, int cascade) 
   __syncthreads();
   auto front_of = [&](int64_t t, int nb) { return nb < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax; };
   auto back_of = [&](int64_t t, int nb) { return nb < kLaneFrameClasses && A.qlen[t] > kLaneFrameQMax; };
-  // in a cascade pass only the tasks the cascade count marked (bit 7) take part
   auto member = [&](int64_t t, int& nb) {
     if (W.bin_of[t] == 0xff) return false;
     nb = W.nbin_of[t];
-    if (nb == 0xff) return false;
-    if (cascade) {
-      if (!(nb & 0x80)) return false;
-      nb &= 0x7f;
-    }
-    return true;
+    return nb != 0xff;
   };
   for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     int nb;
@@ -501,46 +495,7 @@ ext_narrow_scatter_kernel(ExtArgs A, ExtWork W, int64_t per_block, int cascade) 
       p = off[nb] + atomicAdd(&cur[nb], 1u);
     }
     write_desc(A, t, W.desc + p);
-    if (cascade) W.nbin_of[t] = (uint8_t)nb;
   }
-}
-
-// Cascade pass: a task whose frame run of class `from` fell back is routed to
-// class `to` (its plain-kernel flag cleared again); the class lists and their
-// counters are rebuilt for this pass only.
-__global__ void ext_cascade_reset_kernel(ExtWork W) {
-  const int k = threadIdx.x;
-  if (k < kBandClasses) { W.counts[kExtBins + k] = 0; W.cursor[kExtBins + k] = 0; }
-  if (k < 2 * kLaneFrameClasses) W.split[k] = 0;
-  if (k < kBandClasses * kExtSlices) W.fetch[kExtBins * kExtSlices + k] = 0;
-  if (k < kLaneFrameClasses * kRowBuckets) { W.fcount[k] = 0; W.fcursor[k] = 0; }
-}
-static_assert(kLaneFrameClasses * kRowBuckets <= kBandClasses * kExtSlices, "cascade reset threads");
-
-__global__ void __launch_bounds__(256)
-ext_cascade_count_kernel(ExtArgs A, ExtWork W, int from, int to) {
-  __shared__ uint32_t n_to, n_front, fhist[kRowBuckets];
-  if (threadIdx.x == 0) { n_to = 0; n_front = 0; }
-  if (threadIdx.x < kRowBuckets) fhist[threadIdx.x] = 0;
-  __syncthreads();
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    if (W.bin_of[t] == 0xff || W.nbin_of[t] != from || !W.need_plain[t]) continue;
-    W.nbin_of[t] = (uint8_t)(to | 0x80);
-    W.need_plain[t] = 0;
-    atomicAdd(&n_to, 1u);
-    if (to < kLaneFrameClasses && A.qlen[t] <= kLaneFrameQMax) {
-      atomicAdd(&n_front, 1u);
-      atomicAdd(&fhist[row_bucket(A.tlen[t], A.h0[t])], 1u);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (n_to) atomicAdd(&W.counts[kExtBins + to], n_to);
-    if (n_front && to < kLaneFrameClasses) atomicAdd(&W.split[to], n_front);
-  }
-  if (to < kLaneFrameClasses && threadIdx.x < kRowBuckets && fhist[threadIdx.x])
-    atomicAdd(&W.fcount[to * kRowBuckets + threadIdx.x], fhist[threadIdx.x]);
 }
 
 template <int CL, bool E1>
@@ -584,24 +539,10 @@ hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk,
   if (blocks > (int64_t)num_cu * 4) blocks = (int64_t)num_cu * 4;
   hipLaunchKernelGGL(ext_narrow_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, plan);
   const int64_t per_block = (a.n + blocks - 1) / blocks;
-  hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block, 0);
+  hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block);
   ss_fork(wk.ss, st);
   if (a.e_del == 1 && a.e_ins == 1) launch_narrow_classes<true>(k, wk, a.n, num_cu, st, plan);
   else launch_narrow_classes<false>(k, wk, a.n, num_cu, st, plan);
-  ss_join(wk.ss, st);
-  return hipGetLastError();
-}
-
-hipError_t launch_narrow_cascade(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st) {
-  int64_t blocks = (a.n + 255) / 256;
-  if (blocks > (int64_t)num_cu * 4) blocks = (int64_t)num_cu * 4;
-  hipLaunchKernelGGL(ext_cascade_reset_kernel, dim3(1), dim3(kBandClasses * kExtSlices), 0, st, wk);
-  hipLaunchKernelGGL(ext_cascade_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, 0, 1);
-  const int64_t per_block = (a.n + blocks - 1) / blocks;
-  hipLaunchKernelGGL(ext_narrow_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, wk, per_block, 1);
-  ss_fork(wk.ss, st);  // the class runs on a side stream, after the routing above
-  if (a.e_del == 1 && a.e_ins == 1) launch_narrow_class<1, true>(k, wk, a.n, num_cu, st);
-  else launch_narrow_class<1, false>(k, wk, a.n, num_cu, st);
   ss_join(wk.ss, st);
   return hipGetLastError();
 }
