@@ -45,7 +45,7 @@ __host__ __device__ constexpr int frame_of_class(int c) { return frame_lanes(c) 
 // class only where that class is at least 20% cheaper per row than the plain
 // bin of its query length.
 constexpr int kPlainRowPs[kExtBins] = {42, 43, 57, 59, 109, 158, 600, 700, 1000, 2000, 4000};
-constexpr int kFrameRowPs[kBandClasses - 1] = {29, 45, 152, 239};
+constexpr int kFrameRowPs[kBandClasses - 1] = {29, 75, 152, 239};
 static_assert(kExtBins == 11 && kBandClasses == 5, "row cost tables");
 __host__ __device__ constexpr bool narrow_pays(int c, int b) {
   return 10 * kFrameRowPs[c] < 8 * kPlainRowPs[b];
@@ -409,7 +409,10 @@ ext_narrow_count_kernel(ExtArgs A, ExtWork W, BandPlan plan) {
        t += (int64_t)gridDim.x * blockDim.x) {
     const int b = W.bin_of[t];
     if (b == 0xff) continue;
-    const int c = plan.mode == kBandFixed ? plan.fixed : (int)plan.cls[t];
+    int c = plan.mode == kBandFixed ? plan.fixed : (int)plan.cls[t];
+    // a predicted 40-column class whose frame does not pay for this query
+    // length tries the 24-column frame (a failed frame run costs a few rows)
+    if (plan.mode == kBandLearned && c == 1 && !narrow_pays_dev(1, b)) c = 0;
     int nb = 0xff;
     if (c >= 0 && c < kBandClasses - 1 && A.qlen[t] <= kNarrowQMax && (plan.any_gain || narrow_pays_dev(c, b)))
       nb = c;
