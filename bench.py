@@ -125,6 +125,53 @@ def learned_leg(args, eng, ptrs, outs, cells_d, plain_out, plain_cells, n, total
     return res
 
 
+def md_leg(args, eng):
+    """Times the NM / MD:Z / XB:Z pass (the tail of bwa_gen_cigar2, bwa.c:250-343)
+    on the GPU: the final CIGAR of every alignment region of the same workload
+    at reduced scale (its CIGAR pass runs ksw_global2 on the GPU first)."""
+    from bsbolt_amd.workload import CONFIGS, Workload, build_glb_batch, md_inputs
+    cfg = CONFIGS[args.workload].scaled(args.md_scale)
+    wl = Workload(cfg)
+    try:
+        wl.build_ext_batch(lambda t: eng.extend(t, cells=False))
+        out = build_glb_batch(wl, lambda t: eng.global_(t), return_regions=True)
+        idx, rb, re, oq, oq_off, lq, nc, cig = md_inputs(wl, out[7], out[4], out[3])
+        opac, l_pac, crick_l = wl.opac(), wl.l_pac, cfg.genome_len
+    finally:
+        wl.close()
+    n = len(idx)
+    md_slot, xb_slot = 4 * int(lq.max()) + 4 * cig.shape[1] + 16, 4 * int(lq.max()) + 16
+    host = {"opac": opac, "rb": rb, "re": re, "oquery": oq, "oq_off": oq_off, "l_query": lq, "n_cigar": nc,
+            "cigar": cig}
+    keep = {k: eng.to_device(np.ascontiguousarray(v)) for k, v in host.items()}
+    outs = {"md": eng.empty((n, md_slot), np.uint8), "md_len": eng.empty((n,), np.int32),
+            "xb": eng.empty((n, xb_slot), np.uint8), "xb_len": eng.empty((n,), np.int32),
+            "nm": eng.empty((n,), np.int32), "meth": eng.empty((n, 4), np.int32), "status": eng.empty((n,), np.int32)}
+    dev = {k: v.ptr for k, v in {**keep, **outs}.items()}
+    dev.update(n=n, l_pac=l_pac, crick_l=crick_l, cigar_slot=cig.shape[1], md_slot=md_slot, xb_slot=xb_slot)
+    for _ in range(max(1, args.warmup)):
+        eng.md_device(dev)
+    eng.sync()
+    status = outs["status"].numpy()
+    ml, xl = outs["md_len"].numpy(), outs["xb_len"].numpy()
+    eng.timer_start()
+    for _ in range(args.steps):
+        eng.md_device(dev)
+    ms = eng.timer_stop() / args.steps
+    # algorithmic bytes: the 2-bit context window, the read, the CIGAR, the
+    # task fields and the strings and counts written
+    alg = int(((re - rb + 4 + 3) // 4).sum() + lq.astype(np.int64).sum() + 4 * nc.astype(np.int64).sum()
+              + ml.astype(np.int64).sum() + xl.astype(np.int64).sum()) + n * (8 + 8 + 8 + 4 + 4 + 4 * 8)
+    log(f"NM/MD/XB pass: {n} regions, {ms:.3f} ms, status {np.bincount(status + 3, minlength=5).tolist()} "
+        f"(-3..1)")
+    del keep, outs
+    return {"value": n / (ms / 1e3), "unit": "alignments/s", "regions": n, "ms_per_pass": ms,
+            "workload": f"{args.workload} x {args.md_scale}: final CIGAR of every alignment region",
+            "status_ok": int((status == 0).sum()), "status_bridging": int((status == 1).sum()),
+            "roofline_hbm": {"bound": "hbm", "achieved": alg / (ms / 1e3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": alg / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, "algorithmic_bytes": alg}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +182,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-band leg")
+    ap.add_argument("--no-md", action="store_true", help="skip the NM/MD/XB leg")
+    ap.add_argument("--md-scale", type=float, default=0.05, help="fraction of the reads for the NM/MD/XB leg")
     ap.add_argument("--band-model", default=None, help="band model (.npy); default bsbolt_amd/band_model.npy")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
@@ -283,6 +332,8 @@ def main():
                          "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes": alg_bytes},
         "learned_band": learned,
     }
+    if not args.no_md:
+        line["md_layer"] = md_leg(args, eng)
     if learned:  # the north-star ratio: learned-band form over the plain form, and the no-predictor frame form
         learned["speedup_vs_plain"] = ms_step / learned["ms_per_step"]
         if learned.get("fixed24"):

@@ -35,7 +35,7 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
-           "bsx_set_side_streams")
+           "bsx_set_side_streams", "bsx_md_batch_device")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -94,6 +94,18 @@ class _GlbResults(C.Structure):
                 ("cigar_slot", C.c_int64)]
 
 
+class _MdTasks(C.Structure):
+    _fields_ = [("n", C.c_int64), ("l_pac", C.c_int64), ("crick_l", C.c_int64), ("opac", C.c_void_p),
+                ("rb", C.c_void_p), ("re", C.c_void_p), ("oquery", C.c_void_p), ("oq_off", C.c_void_p),
+                ("l_query", C.c_void_p), ("n_cigar", C.c_void_p), ("cigar", C.c_void_p), ("cigar_slot", C.c_int64)]
+
+
+class _MdResults(C.Structure):
+    _fields_ = [("md", C.c_void_p), ("md_slot", C.c_int64), ("md_len", C.c_void_p), ("xb", C.c_void_p),
+                ("xb_slot", C.c_int64), ("xb_len", C.c_void_p), ("nm", C.c_void_p), ("meth", C.c_void_p),
+                ("status", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -150,6 +162,8 @@ def lib():
         L.bsx_kernel_times.argtypes = [P, P, C.c_int, P]
         L.bsx_ext_bin_limits.argtypes = [P, C.c_int]
         L.bsx_set_side_streams.argtypes = [P, C.c_int]
+        L.bsx_md_batch_device.argtypes = [P, C.POINTER(_MdTasks), C.POINTER(_MdResults)]
+        L.bsx_md_batch_device.restype = C.c_int
         for f in ("bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits", "bsx_set_side_streams"):
             getattr(L, f).restype = C.c_int
         for f in ("bsx_set_band_policy", "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get",
@@ -461,6 +475,46 @@ class Engine:
                                    ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")])
         rs = _GlbResults(dev.get("score"), dev.get("n_cigar"), dev.get("cigar"), dev.get("cigar_slot", 0))
         self._check(self.L.bsx_global_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
+
+
+    # ---- NM / MD:Z / XB:Z (bwa_gen_cigar2's tail, bwa.c:250-343) ----
+    def md_device(self, dev):
+        """Device-memory form: `dev` maps the bsx_md_tasks / bsx_md_results field
+        names to device addresses (and n, l_pac, crick_l and the slot sizes to ints)."""
+        tk = _MdTasks(dev["n"], dev["l_pac"], dev["crick_l"], *[dev[k] for k in
+                      ("opac", "rb", "re", "oquery", "oq_off", "l_query", "n_cigar", "cigar")], dev["cigar_slot"])
+        rs = _MdResults(dev["md"], dev["md_slot"], dev["md_len"], dev["xb"], dev["xb_slot"], dev["xb_len"],
+                        dev["nm"], dev["meth"], dev["status"])
+        self._check(self.L.bsx_md_batch_device(self.ctx, C.byref(tk), C.byref(rs)))
+
+    def md(self, opac, l_pac, crick_l, rb, re, oquery, oq_off, l_query, n_cigar, cigar, md_slot=None, xb_slot=None):
+        """Host arrays in, host arrays out: NM, MD:Z and XB:Z of each task's
+        CIGAR (cigar is [n, slot] uint32 as global_ returns it).  Returns a dict
+        with md / xb (lists of str), nm, meth [n, 4], status."""
+        n = len(rb)
+        if md_slot is None:
+            md_slot = int(4 * (np.max(l_query) if n else 1) + 4 * cigar.shape[1] + 16)
+        if xb_slot is None:
+            xb_slot = int(4 * (np.max(l_query) if n else 1) + 16)
+        host = {"opac": np.ascontiguousarray(opac, np.uint8), "rb": np.ascontiguousarray(rb, np.int64),
+                "re": np.ascontiguousarray(re, np.int64), "oquery": np.ascontiguousarray(oquery, np.uint8),
+                "oq_off": np.ascontiguousarray(oq_off, np.uint64), "l_query": np.ascontiguousarray(l_query, np.int32),
+                "n_cigar": np.ascontiguousarray(n_cigar, np.int32),
+                "cigar": np.ascontiguousarray(cigar, np.uint32).reshape(n, -1)}
+        keep = {k: self.to_device(v) for k, v in host.items()}
+        out = {"md": self.empty((n, md_slot), np.uint8), "md_len": self.empty((n,), np.int32),
+               "xb": self.empty((n, xb_slot), np.uint8), "xb_len": self.empty((n,), np.int32),
+               "nm": self.empty((n,), np.int32), "meth": self.empty((n, 4), np.int32),
+               "status": self.empty((n,), np.int32)}
+        dev = {k: v.ptr for k, v in {**keep, **out}.items()}
+        dev.update(n=n, l_pac=int(l_pac), crick_l=int(crick_l), cigar_slot=int(host["cigar"].shape[1]),
+                   md_slot=md_slot, xb_slot=xb_slot)
+        self.md_device(dev)
+        self.sync()
+        h = {k: v.numpy() for k, v in out.items()}
+        return {"md": [bytes(h["md"][i, :h["md_len"][i]]).decode() for i in range(n)],
+                "xb": [bytes(h["xb"][i, :h["xb_len"][i]]).decode() for i in range(n)],
+                "nm": h["nm"], "meth": h["meth"], "status": h["status"]}
 
 
 def _glb_subset(t, a, b):

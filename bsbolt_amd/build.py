@@ -8,7 +8,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsx.so")
 WL_LIB = os.path.join(HERE, "libbsxwl.so")
-SOURCES = ["api.hip", "extend.hip", "lane.hip", "narrow.hip", "mlp.hip", "global.hip"]
+SOURCES = ["api.hip", "extend.hip", "lane.hip", "narrow.hip", "mlp.hip", "global.hip", "mdxb.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",

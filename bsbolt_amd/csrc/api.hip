@@ -661,6 +661,29 @@ void bsx_set_global_zslot(bsx_ctx* c, int64_t bytes) {
   if (c && bytes >= 64) c->zslot = bytes;
 }
 
+int bsx_md_batch_device(bsx_ctx* c, const bsx_md_tasks* t, bsx_md_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  if (t->n < 0 || t->n > INT_MAX) return fail(c, BSX_EINVAL, "n must be in [0, 2^31-1]");
+  if (t->n == 0) return BSX_OK;
+  if (!t->opac || !t->rb || !t->re || !t->oquery || !t->oq_off || !t->l_query || !t->n_cigar || !t->cigar)
+    return fail(c, BSX_EINVAL, "task arrays must not be NULL");
+  if (!r->md || !r->md_len || !r->xb || !r->xb_len || !r->nm || !r->meth || !r->status)
+    return fail(c, BSX_EINVAL, "result arrays must not be NULL");
+  if (t->l_pac < 1 || t->crick_l < 0 || t->crick_l > t->l_pac || t->cigar_slot < 1 || r->md_slot < 1 ||
+      r->xb_slot < 1 || r->md_slot > INT_MAX || r->xb_slot > INT_MAX)
+    return fail(c, BSX_EINVAL, "need l_pac >= 1, 0 <= crick_l <= l_pac, slots >= 1");
+  (void)hipSetDevice(c->device);
+  MdArgs a{};
+  a.n = t->n; a.l_pac = t->l_pac; a.crick_l = t->crick_l; a.opac = t->opac;
+  a.rb = t->rb; a.re = t->re; a.oquery = t->oquery; a.oq_off = t->oq_off; a.l_query = t->l_query;
+  a.n_cigar = t->n_cigar; a.cigar = t->cigar; a.cigar_slot = t->cigar_slot;
+  a.md = r->md; a.md_slot = r->md_slot; a.md_len = r->md_len;
+  a.xb = r->xb; a.xb_slot = r->xb_slot; a.xb_len = r->xb_len;
+  a.nm = r->nm; a.meth = r->meth; a.status = r->status;
+  HIPCHK(c, launch_md(a, c->num_cu, c->stream), "md launch");
+  return BSX_OK;
+}
+
 int bsx_global_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t,
                             bsx_glb_results* r) {
   if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");

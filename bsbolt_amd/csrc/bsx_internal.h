@@ -217,6 +217,27 @@ struct GlbWork {
   int64_t zslot; // bytes per slot
 };
 
+// NM / MD:Z / XB:Z of CIGARs (bwa_gen_cigar2's tail, bwa.c:250-343); fields as
+// bsx_md_tasks / bsx_md_results in include/bsx.h.
+struct MdArgs {
+  int64_t n, l_pac, crick_l;
+  const uint8_t* opac;
+  const int64_t *rb, *re;
+  const uint8_t* oquery;
+  const uint64_t* oq_off;
+  const int32_t* l_query;
+  const int32_t* n_cigar;
+  const uint32_t* cigar;
+  int64_t cigar_slot;
+  char* md;
+  int64_t md_slot;
+  int32_t* md_len;
+  char* xb;
+  int64_t xb_slot;
+  int32_t *xb_len, *nm, *meth, *status;
+};
+hipError_t launch_md(const MdArgs& a, int num_cu, hipStream_t st);
+
 // Blocks per CU of the global kernel; the scratch holds one slot per group.
 constexpr int kGlbBlocksPerCU = 4;
 int64_t glb_groups(int num_cu);

@@ -100,7 +100,7 @@ struct Chain {
   int last_out[6]{};
 };
 
-struct Read { std::vector<uint8_t> seq; };
+struct Read { std::vector<uint8_t> seq, oseq; };  // converted as aligned; as sequenced
 
 }  // namespace
 
@@ -277,6 +277,7 @@ int64_t wl_simulate(wl_ctx* c, const uint8_t* g, uint64_t seed, int64_t n_frag, 
           rmap.push_back(fmap[k]);
         }
       }
+      std::vector<uint8_t> ord(rd);  // the read as sequenced (bseq1_t.oseq)
       // aligner-side conversion: read 1 C->T, read 2 G->A
       for (auto& b : rd) {
         if (m == 0 && b == 1) b = 3;
@@ -292,7 +293,7 @@ int64_t wl_simulate(wl_ctx* c, const uint8_t* g, uint64_t seed, int64_t n_frag, 
         return strand == 0 ? 2 * l_pac - 1 - x : G + x;
       };
       const int read_id = (int)c->reads.size();
-      c->reads.push_back(Read{rd});
+      c->reads.push_back(Read{rd, std::move(ord)});
       // seeds: maximal runs along the true path where consecutive read bases map
       // to consecutive reference positions and the converted bases agree
       int run_q = -1;
@@ -335,6 +336,10 @@ int64_t wl_simulate(wl_ctx* c, const uint8_t* g, uint64_t seed, int64_t n_frag, 
 int32_t wl_read_len(const wl_ctx* c, int64_t i) { return (int32_t)c->reads[i].seq.size(); }
 void wl_read_seq(const wl_ctx* c, int64_t i, uint8_t* out) {
   memcpy(out, c->reads[i].seq.data(), c->reads[i].seq.size());
+}
+// the read as sequenced, before the in-silico conversion (bwa.c:95-133 keeps it as oseq)
+void wl_read_oseq(const wl_ctx* c, int64_t i, uint8_t* out) {
+  memcpy(out, c->reads[i].oseq.data(), c->reads[i].oseq.size());
 }
 
 // ---------------- mem_chain2aln rounds ----------------

@@ -92,6 +92,7 @@ def wl():
         L.wl_read_len.restype = C.c_int32
         L.wl_read_len.argtypes = [_P, C.c_int64]
         L.wl_read_seq.argtypes = [_P, C.c_int64, _P]
+        L.wl_read_oseq.argtypes = [_P, C.c_int64, _P]
         L.wl_chains_init.restype = C.c_int64
         L.wl_chains_init.argtypes = [_P, C.c_int64, _P, _P, _P, _P, _P]
         L.wl_round_emit.restype = C.c_int64
@@ -178,6 +179,18 @@ class Workload:
         wl().wl_read_seq(self.ctx, i, _p(out))
         return out
 
+    def oread(self, i):
+        """Read i as sequenced (before the aligner's in-silico conversion)."""
+        n = wl().wl_read_len(self.ctx, i)
+        out = np.zeros(n, np.uint8)
+        wl().wl_read_oseq(self.ctx, i, _p(out))
+        return out
+
+    def opac(self):
+        """The unconverted reference in the .pac 2-bit layout, both copies
+        (bwtindex.c:303-304 writes it as .opac): l_pac/4+1 bytes."""
+        return pack2(np.concatenate([self.genome, self.genome]))
+
     def get_seq(self, beg, end):
         out = np.zeros(abs(end - beg) + 1, np.uint8)
         n = wl().wl_get_seq(self.ctx, beg, end, _p(out))
@@ -246,7 +259,7 @@ def cigar_band(w_, l_query, rlen, mat0=1, o_del=6, e_del=1, o_ins=6, e_ins=1):
     return np.maximum(w, d + 3)
 
 
-def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only=True):
+def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only=True, return_regions=False):
     """CIGAR pass over every alignment region of a workload, in rounds, as
     mem_reg2aln (bwamem.c:1197-1233) and bwa_gen_cigar2 (bwa.c:199-248) run it:
     band w2 = max of the inferred deletion/insertion bands (capped by the
@@ -258,7 +271,8 @@ def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only
 
     `global_fn(GlbTasks) -> (score, n_cigar, cigar[n, slot])` runs one round.
     Returns (all issued tasks as one GlbTasks, their scores, n_cigar, CIGAR
-    lists, region index per task, round per task, final score per region)."""
+    lists, region index per task, round per task, final score per region), and
+    with return_regions a dict of per-region arrays read, qb, qe, rb, re."""
     o = dict(DEFAULT_OPT) if opt is None else opt
     mat = fill_matrix(o["a"], o["b"]) if mat is None else np.asarray(mat, np.int8).reshape(25)
     regs = []
@@ -337,8 +351,66 @@ def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only
         active[idx[cont]] = True
     batch = concat_glb(parts)
     cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
-    return (batch, cat(scores, np.int32), cat(ncig, np.int32), cigs, cat(owner, np.int64), cat(rnd, np.int32),
-            final)
+    out = (batch, cat(scores, np.int32), cat(ncig, np.int32), cigs, cat(owner, np.int64), cat(rnd, np.int32),
+           final)
+    if return_regions:
+        reg = {"read": np.array([r[0] for r in regs], np.int64)}
+        for k, c in (("qb", 0), ("qe", 1), ("rb", 2), ("re", 3)):
+            reg[k] = R[:, c].copy()
+        out = out + (reg,)
+    return out
+
+
+def md_inputs(wl, regions, owner, cigars):
+    """Inputs of the NM / MD:Z / XB:Z pass over the regions of a CIGAR pass
+    (build_glb_batch with return_regions): one task per region with its final
+    CIGAR -- the last try's, or l_query M for a region scored without DP
+    (bwa.c:222-230).  Returns (region index, rb, re, oquery, oq_off, l_query,
+    n_cigar, cigar[n, slot]).  The read goes in as mem_reg2aln passes it to
+    bwa_gen_cigar2 (bwamem.c:1228): from the read's first base, with
+    l_query = qe - qb -- the oquery argument is not offset by qb, unlike the
+    converted query."""
+    last = {}
+    for t, g in enumerate(owner):
+        last[int(g)] = t
+    idx, cg = [], []
+    for g in range(len(regions["rb"])):
+        rb, re = int(regions["rb"][g]), int(regions["re"][g])
+        lq = int(regions["qe"][g] - regions["qb"][g])
+        if g in last:
+            idx.append(g); cg.append(np.asarray(cigars[last[g]], np.uint32))
+        elif lq == re - rb and not (rb < wl.l_pac < re) and re <= 2 * wl.l_pac:
+            idx.append(g); cg.append(np.array([lq << 4], np.uint32))
+    idx = np.array(idx, np.int64)
+    rb = regions["rb"][idx].astype(np.int64)
+    re = regions["re"][idx].astype(np.int64)
+    l_query = (regions["qe"] - regions["qb"])[idx].astype(np.int32)
+    reads, parts = {}, []
+    for t, g in enumerate(idx):
+        i = int(regions["read"][g])
+        if i not in reads:
+            reads[i] = wl.oread(i)
+        parts.append(reads[i][:int(l_query[t])])
+    n = len(idx)
+    oq_off = np.zeros(n, np.uint64)
+    if n:
+        oq_off[1:] = np.cumsum([len(p) for p in parts])[:-1]
+    slot = max(1, max((len(c) for c in cg), default=1))
+    cig = np.zeros((n, slot), np.uint32)
+    for t, c in enumerate(cg):
+        cig[t, :len(c)] = c
+    return (idx, rb, re, np.concatenate(parts).astype(np.uint8) if parts else np.zeros(1, np.uint8), oq_off,
+            l_query, np.array([len(c) for c in cg], np.int32), cig)
+
+
+def pack2(codes):
+    """2-bit forward-only layout of the .pac files (first base in the top bits),
+    len/4+1 bytes as bwa.c:430-431 reads them."""
+    codes = np.asarray(codes, np.uint8)
+    out = np.zeros(len(codes) // 4 + 1, np.uint8)
+    idx = np.arange(len(codes))
+    np.bitwise_or.at(out, idx >> 2, ((codes & 3) << ((~idx & 3) << 1)).astype(np.uint8))
+    return out
 
 
 def fill_matrix(a=1, b=4):

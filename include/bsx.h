@@ -6,6 +6,8 @@
  * bit-exact HIP kernels:
  *   ksw_extend2  ksw.h:108, ksw.c:380-479   -> bsx_extend_batch / bsx_extend_batch_device
  *   ksw_global2  ksw.h:84,  ksw.c:504-606   -> bsx_global_batch / bsx_global_batch_device
+ * and the NM/MD/XB strings bwa_gen_cigar2 appends to a CIGAR (bwa.c:250-343)
+ *                                           -> bsx_md_batch_device
  * Their callers in the reference are mem_chain2aln (bwamem.c:735,763) and
  * bwa_gen_cigar2 (bwa.c:248).  Each task of a batch carries exactly the
  * arguments one reference call takes; each result carries exactly its outputs.
@@ -133,6 +135,53 @@ int bsx_global_batch_device(bsx_ctx *ctx, const bsx_scoring *sc,
  * needs min(qlen, 2w+1) * tlen bytes; larger ones are rejected (BSX_EINVAL).
  * The context holds (#CU * 64) such slots of device memory. */
 void bsx_set_global_zslot(bsx_ctx *ctx, int64_t bytes);
+
+/* ---- NM, MD:Z and XB:Z of CIGARs (the tail of bwa_gen_cigar2, bwa.c:250-343) ----
+ * For each task: the region [rb, re) of the doubled reference a CIGAR was made
+ * for (rb < re, not bridging l_pac), the read as sequenced (before the
+ * in-silico conversion; codes 0..4, read order) and that CIGAR as
+ * bsx_global_batch wrote it (computed on the reversed sequences when
+ * rb >= l_pac, bwa.c:216-221).  opac is the unconverted reference in the .pac
+ * 2-bit layout (both copies, l_pac bases, bwa.c:430-431); crick_l is the first
+ * position of the Crick copy (bntseq.c:139).  Outputs per task: the MD:Z value
+ * (md[t*md_slot ..], md_len[t] bytes, no terminator), the XB:Z value (xb, likewise),
+ * NM and meth[4*t ..] = CpG methylated, CpG unmethylated, CH methylated, CH
+ * unmethylated (bwa_gen_cigar2's cg_meth, cg_unmeth, ch_meth, ch_unmeth).
+ * status[t]: 0 ok; 1 the context window opac[rb-2, re+2) bridges the strands
+ * (the reference appends nothing; nm = -1); -1 an output slot is too small;
+ * -2 the walk would leave the window (within two bases of the reference ends,
+ * where the reference reads past its buffer); -3 the region is not one the
+ * reference makes a CIGAR for (rb < 0, rb >= re, re > 2*l_pac) or the CIGAR has
+ * more than cigar_slot ops, an op other than M/I/D, or consumes more than
+ * l_query read bases.  Tasks with status != 0 have
+ * nm = -1 and empty strings; the call still returns BSX_OK.  Device memory
+ * only; queued on the context's stream. */
+typedef struct {
+	int64_t n;
+	int64_t l_pac, crick_l;
+	const uint8_t  *opac;
+	const int64_t  *rb, *re;
+	const uint8_t  *oquery;
+	const uint64_t *oq_off;
+	const int32_t  *l_query;
+	const int32_t  *n_cigar;
+	const uint32_t *cigar;
+	int64_t         cigar_slot;
+} bsx_md_tasks;
+
+typedef struct {
+	char    *md;
+	int64_t  md_slot;
+	int32_t *md_len;
+	char    *xb;
+	int64_t  xb_slot;
+	int32_t *xb_len;
+	int32_t *nm;
+	int32_t *meth;
+	int32_t *status;
+} bsx_md_results;
+
+int bsx_md_batch_device(bsx_ctx *ctx, const bsx_md_tasks *tasks, bsx_md_results *res);
 
 /* ---- device memory and timing on the context's device/stream ----
  * Helpers for callers that hold their batches in device memory without a

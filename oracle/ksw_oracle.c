@@ -13,6 +13,8 @@
  *   orc_global_band     bwa.c:231-241          (band choice in bwa_gen_cigar2)
  *   orc_bs_convert      bs_helpers.cpp:18-31   (bsConversion)
  *   orc_nt4             bntseq.c:48-65         (nst_nt4_table)
+ *   orc_gen_md          bwa.c:180-197,250-343  (NM, MD and XB:Z of bwa_gen_cigar2,
+ *                                               getMethylationContext)
  * Parity of each function is pinned by tests against golden vectors produced by
  * the compiled reference (oracle/_ref) and by closed-form cases.
  */
@@ -284,6 +286,138 @@ int64_t orc_get_seq(int64_t l_pac, const uint8_t *pac, int64_t beg, int64_t end,
 			out[n++] = pac[k >> 2] >> ((~k & 3) << 1) & 3;
 	}
 	return n;
+}
+
+/* Decimal digits of a non-negative int appended to buf (kputw, kstring.h:62-76). */
+static int put_int(char *buf, int l, int cap, int v)
+{
+	char d[12];
+	int n = 0;
+	if (v == 0) d[n++] = '0';
+	for (; v > 0; v /= 10) d[n++] = (char)('0' + v % 10);
+	if (l + n > cap) return -1;
+	while (n > 0) buf[l++] = d[--n];
+	return l;
+}
+
+/* bwa.c:180-197: methylation context of a C (or G on the Crick copy): CpG when
+ * the next base is G, CHG when the one after is, CHH otherwise; upper case
+ * when the read kept the base (methylated). */
+static char meth_context(int b1, int b2, int b3, int meth, int *cg, int *ch)
+{
+	if (b1 == 1 && b2 == 2) { ++*cg; return meth ? 'X' : 'x'; }
+	if (b1 == 1 && b3 == 2) { ++*ch; return meth ? 'Y' : 'y'; }
+	++*ch;
+	return meth ? 'Z' : 'z';
+}
+
+/* bwa.c:250-343: the NM, MD and XB:Z strings bwa_gen_cigar2 appends after a
+ * CIGAR it produced for the region [rb, re) of the doubled reference.  opac is
+ * the unconverted reference in the same 2-bit layout as pac (both copies,
+ * l_pac bases); oquery the read as sequenced (before the in-silico conversion),
+ * in read order; cigar the ksw_global2 CIGAR (computed on the reversed
+ * sequences when rb >= l_pac, bwa.c:216-221).  The context window is opac
+ * [rb-2, re+2) fetched by bns_get_seq (clamped at the reference ends exactly as
+ * there) and reversed when rb >= l_pac, as is the read.  Writes md[] (MD:Z
+ * value), xb[] (XB:Z value), *NM and counts = {CpG methylated, CpG
+ * unmethylated, CH methylated, CH unmethylated}.  Returns 0; 1 when the window
+ * bridges the strands (the reference stops there: no MD, NM stays -1); -1 when
+ * md_cap or xb_cap is too small; -2 when the walk would index the window out of
+ * range (possible only within two bases of the reference ends, where the
+ * reference reads past its buffer); -3 when [rb, re) is not a region the
+ * reference makes a CIGAR for (rb < 0, rb >= re, re > 2*l_pac) or the CIGAR has
+ * an op other than M/I/D or consumes more than l_query read bases. */
+int orc_gen_md(int64_t l_pac, const uint8_t *opac, int64_t rb, int64_t re, int64_t crick_l, int l_query,
+               const uint8_t *oquery, int n_cigar, const uint32_t *cigar, char *md, int md_cap, int *md_len,
+               char *xb, int xb_cap, int *xb_len, int *NM, int counts[4])
+{
+	*NM = -1; *md_len = 0; *xb_len = 0;
+	counts[0] = counts[1] = counts[2] = counts[3] = 0;
+	{  /* a CIGAR of M/I/D ops consuming at most the read (ksw_global2 makes no other) */
+		int64_t qsum = 0;
+		for (int k = 0; k < n_cigar; ++k) {
+			const int op = cigar[k] & 0xf;
+			if (op == 0 || op == 1) qsum += cigar[k] >> 4;
+			else if (op != 2) return -3;
+		}
+		if (n_cigar < 0 || qsum > l_query || rb < 0 || rb >= re || re > 2 * l_pac) return -3;
+	}
+	const int reverse = rb > l_pac ? 1 : 0;  /* bwa.c:212: strictly greater */
+	const int is_crick = (rb >= crick_l && rb < l_pac) || rb >= crick_l + l_pac;
+	const int ref_base = is_crick ? 2 : 1, methyl_base = is_crick ? 0 : 3;
+	uint8_t *cs = (uint8_t *)malloc((size_t)(re - rb + 4));
+	int64_t clen = orc_get_seq(l_pac, opac, rb - 2, re + 2, cs);
+	if (!clen) { free(cs); return 1; }
+	uint8_t *oq = (uint8_t *)malloc((size_t)(l_query > 0 ? l_query : 1));
+	for (int i = 0; i < l_query; ++i) oq[i] = oquery[i];
+	if (rb >= l_pac) {
+		for (int64_t i = 0; i < clen >> 1; ++i) { uint8_t t = cs[i]; cs[i] = cs[clen - 1 - i]; cs[clen - 1 - i] = t; }
+		for (int i = 0; i < l_query >> 1; ++i) { uint8_t t = oq[i]; oq[i] = oq[l_query - 1 - i]; oq[l_query - 1 - i] = t; }
+	}
+	const char *i2b = rb < l_pac ? "ACGTN" : "TGCAN";
+	int ret = 0, ml = 0, xl = 0, x = 0, y = 0, u = 0, n_mm = 0, n_gap = 0, meth_pos = 0;
+	int cgm = 0, cgu = 0, chm = 0, chu = 0;
+#define CS(k) ((k) >= 0 && (k) < clen ? (int)cs[k] : (ret = -2, 4))
+#define PUTW(v) do { ml = put_int(md, ml, md_cap, (v)); if (ml < 0) { ret = -1; goto out; } } while (0)
+#define PUTC(c) do { if (ml + 1 > md_cap) { ret = -1; goto out; } md[ml++] = (char)(c); } while (0)
+#define XPUTW(v) do { xl = put_int(xb, xl, xb_cap, (v)); if (xl < 0) { ret = -1; goto out; } } while (0)
+#define XPUTC(c) do { if (xl + 1 > xb_cap) { ret = -1; goto out; } xb[xl++] = (char)(c); } while (0)
+	for (int k = 0; k < n_cigar; ++k) {
+		const int op = cigar[k] & 0xf, len = (int)(cigar[k] >> 4);
+		if (op == 0) {
+			for (int i = 0; i < len; ++i) {
+				const int c = CS(y + i + 2);
+				if (c == ref_base) {
+					int b1, b2, b3;
+					if (!reverse && !is_crick) b1 = CS(y + i + 2), b2 = CS(y + i + 3), b3 = CS(y + i + 4);
+					else if (!reverse && is_crick) b1 = CS(y + i + 1), b2 = CS(y + i + 2), b3 = CS(y + i + 3);
+					else if (reverse && !is_crick) b1 = CS(y + i + 2), b2 = CS(y + i + 1), b3 = CS(y + i);
+					else b1 = CS(y + i + 3), b2 = CS(y + i + 2), b3 = CS(y + i + 1);
+					const int o = oq[x + i];
+					if (o == ref_base || o == methyl_base) {
+						const char m = o == ref_base ? meth_context(b1, b2, b3, 1, &cgm, &chm)
+						                             : meth_context(b1, b2, b3, 0, &cgu, &chu);
+						if (meth_pos > 0) XPUTW(meth_pos);
+						XPUTC(m);
+						meth_pos = 0, ++u;
+					} else {
+						PUTW(u); PUTC(i2b[c]);
+						++n_mm; ++meth_pos; u = 0;
+					}
+				} else if (oq[x + i] != c) {
+					PUTW(u); PUTC(i2b[c]);
+					++n_mm; ++meth_pos; u = 0;
+				} else {
+					++u; ++meth_pos;
+				}
+				if (ret == -2) goto out;
+			}
+			x += len; y += len;
+		} else if (op == 2) {
+			if (k > 0 && k < n_cigar - 1) {  /* not for a leading or trailing D */
+				PUTW(u); PUTC('^');
+				for (int i = 0; i < len; ++i) { const int c = CS(y + i + 2); PUTC(i2b[c]); }
+				if (ret == -2) goto out;
+				u = 0; n_gap += len;
+			}
+			y += len;
+		} else if (op == 1) {
+			x += len; n_gap += len; meth_pos += len;
+		}
+	}
+	PUTW(u);
+	if (meth_pos > 0) XPUTW(meth_pos);
+	*NM = n_mm + n_gap;
+	*md_len = ml; *xb_len = xl;
+	counts[0] = cgm; counts[1] = cgu; counts[2] = chm; counts[3] = chu;
+out:
+#undef CS
+#undef PUTW
+#undef PUTC
+#undef XPUTW
+#undef XPUTC
+	free(cs); free(oq);
+	return ret;
 }
 
 /* bs_helpers.cpp:18-31: in-silico bisulfite conversion of ASCII bases.

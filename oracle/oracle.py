@@ -14,9 +14,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libkswref.so")
+BWAREF_PATH = os.path.join(HERE, "_ref", "libbwaref.so")
 
 _lib = None
 _ref = None
+_bwaref = None
 
 _P = C.c_void_p
 _I = C.c_int
@@ -52,6 +54,8 @@ def lib():
         L.orc_nt4.restype = _I
         L.orc_nt4.argtypes = [_I]
         L.orc_bs_convert.argtypes = [C.c_char_p, _I, _I]
+        L.orc_gen_md.restype = _I
+        L.orc_gen_md.argtypes = [_I64, _P, _I64, _I64, _I64, _I, _P, _I, _P, _P, _I, _P, _P, _I, _P, _P, _P]
         _lib = L
     return _lib
 
@@ -74,6 +78,61 @@ def ref():
         R.ksw_global2.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _P, _P]
         _ref = R
     return _ref
+
+
+def bwaref():
+    """The reference's bwa.c (bwa_gen_cigar2) with the sources it links, compiled by
+    oracle/Makefile (container only)."""
+    global _bwaref
+    if _bwaref is None:
+        if not os.path.exists(BWAREF_PATH):
+            raise RuntimeError(f"compiled reference missing: {BWAREF_PATH}")
+        R = C.CDLL(BWAREF_PATH)
+        R.bwa_gen_cigar2.restype = C.POINTER(C.c_uint32)
+        R.bwa_gen_cigar2.argtypes = [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _I, _P, _I64, _I64, _P, _P, _P, _I64, _P,
+                                     _P, _P, _P, _P]
+        _bwaref = R
+    return _bwaref
+
+
+def ref_gen_cigar2(mat, o_del, e_del, o_ins, e_ins, w, l_pac, pac, opac, query, rb, re, crick_l, oquery):
+    """One bwa_gen_cigar2 call of the compiled reference (bwa.c:199-351).  Returns
+    None when it returns no CIGAR, else dict(score, cigar [uint32], NM, md, xb,
+    meth [cg_meth, cg_unmeth, ch_meth, ch_unmeth]); md and xb are None when the
+    reference appended no strings (its context window bridged the strands)."""
+    R = bwaref()
+    q = np.array(query, np.uint8)
+    oq = np.array(oquery, np.uint8)
+    mat = np.ascontiguousarray(mat, np.int8)
+    sc, nc, nm = C.c_int(0), C.c_int(0), C.c_int(0)
+    m = [C.c_int(0) for _ in range(4)]
+    p = R.bwa_gen_cigar2(mat.ctypes.data, o_del, e_del, o_ins, e_ins, w, l_pac, pac.ctypes.data, opac.ctypes.data,
+                         len(q), q.ctypes.data, rb, re, C.byref(sc), C.byref(nc), C.byref(nm), crick_l,
+                         oq.ctypes.data, *[C.byref(x) for x in m])
+    if not p:
+        return None
+    n = nc.value
+    cig = np.array([p[k] for k in range(n)], np.uint32)
+    md = xb = None
+    if nm.value >= 0:
+        raw = C.string_at(C.addressof(p.contents) + 4 * n)
+        md_b, _, rest = raw.partition(b"\t")
+        md, xb = md_b.decode(), rest[len(b"XB:Z:"):].decode()
+    lib().orc_free(C.cast(p, _P))
+    return dict(score=sc.value, cigar=cig, NM=nm.value, md=md, xb=xb, meth=[x.value for x in m])
+
+
+def gen_md(l_pac, opac, rb, re, crick_l, oquery, cigar, md_cap=4096, xb_cap=4096):
+    """orc_gen_md for one task: (status, md, xb, NM, meth[4])."""
+    oq = np.ascontiguousarray(oquery, np.uint8)
+    cg = np.ascontiguousarray(cigar, np.uint32)
+    md = C.create_string_buffer(md_cap)
+    xb = C.create_string_buffer(xb_cap)
+    ml, xl, nm = C.c_int(0), C.c_int(0), C.c_int(0)
+    cnt = np.zeros(4, np.int32)
+    r = lib().orc_gen_md(l_pac, opac.ctypes.data, rb, re, crick_l, len(oq), oq.ctypes.data, len(cg), cg.ctypes.data,
+                         md, md_cap, C.byref(ml), xb, xb_cap, C.byref(xl), C.byref(nm), cnt.ctypes.data)
+    return r, md.raw[:ml.value].decode(), xb.raw[:xl.value].decode(), nm.value, [int(x) for x in cnt]
 
 
 def _u8(a):
