@@ -139,7 +139,12 @@ def md_leg(args, eng):
         opac, l_pac, crick_l = wl.opac(), wl.l_pac, cfg.genome_len
     finally:
         wl.close()
-    n = len(idx)
+    # the regions' tasks repeated md_tile times: a batch the size of a full
+    # chunk of reads, so the pass is timed at full occupancy
+    reps = max(1, args.md_tile)
+    rb, re, oq_off, lq, nc = (np.tile(x, reps) for x in (rb, re, oq_off, lq, nc))
+    cig = np.tile(cig, (reps, 1))
+    n = len(rb)
     md_slot, xb_slot = 4 * int(lq.max()) + 4 * cig.shape[1] + 16, 4 * int(lq.max()) + 16
     host = {"opac": opac, "rb": rb, "re": re, "oquery": oq, "oq_off": oq_off, "l_query": lq, "n_cigar": nc,
             "cigar": cig}
@@ -162,11 +167,28 @@ def md_leg(args, eng):
     # task fields and the strings and counts written
     alg = int(((re - rb + 4 + 3) // 4).sum() + lq.astype(np.int64).sum() + 4 * nc.astype(np.int64).sum()
               + ml.astype(np.int64).sum() + xl.astype(np.int64).sum()) + n * (8 + 8 + 8 + 4 + 4 + 4 * 8)
-    log(f"NM/MD/XB pass: {n} regions, {ms:.3f} ms, status {np.bincount(status + 3, minlength=5).tolist()} "
-        f"(-3..1)")
+    log(f"NM/MD/XB pass: {n} regions ({len(idx)} distinct x {reps}), {ms:.3f} ms, "
+        f"status {np.bincount(status + 3, minlength=5).tolist()} (-3..1)")
+    cpu = None
+    if not args.no_cpu:  # the oracle's restatement on the host threads, over the distinct regions
+        from oracle import oracle as O
+        m = len(idx)
+        ref, dt = O.gen_md_batch(l_pac, opac, crick_l, rb[:m], re[:m], oq, oq_off[:m], lq[:m], nc[:m], cig[:m],
+                                 md_slot, xb_slot, nthreads=args.cpu_threads)
+        gmd, gxb = outs["md"].numpy()[:m], outs["xb"].numpy()[:m]
+        mism = int((ref["status"] != status[:m]).sum() + (ref["nm"] != outs["nm"].numpy()[:m]).sum()
+                   + (ref["meth"] != outs["meth"].numpy()[:m]).any(axis=1).sum())
+        mism += sum(ref["md"][i] != bytes(gmd[i, :ml[i]]).decode() or ref["xb"][i] != bytes(gxb[i, :xl[i]]).decode()
+                    for i in range(m))
+        cpu = {"value": m / dt, "unit": "alignments/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"the {m} distinct regions, oracle restatement of bwa.c:250-343", "seconds": dt,
+               "mismatches_vs_gpu": int(mism)}
+        log(f"NM/MD/XB CPU restatement: {m / dt / 1e6:.2f} M regions/s on {args.cpu_threads} threads, "
+            f"{mism} mismatches vs GPU")
     del keep, outs
-    return {"value": n / (ms / 1e3), "unit": "alignments/s", "regions": n, "ms_per_pass": ms,
-            "workload": f"{args.workload} x {args.md_scale}: final CIGAR of every alignment region",
+    return {"value": n / (ms / 1e3), "unit": "alignments/s", "regions": n, "ms_per_pass": ms, "cpu_baseline": cpu,
+            "workload": f"{args.workload} x {args.md_scale}: final CIGAR of every alignment region "
+                        f"({len(idx)} regions, batch of {reps} copies)",
             "status_ok": int((status == 0).sum()), "status_bridging": int((status == 1).sum()),
             "roofline_hbm": {"bound": "hbm", "achieved": alg / (ms / 1e3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": alg / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, "algorithmic_bytes": alg}}
@@ -184,6 +206,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-band leg")
     ap.add_argument("--no-md", action="store_true", help="skip the NM/MD/XB leg")
     ap.add_argument("--md-scale", type=float, default=0.05, help="fraction of the reads for the NM/MD/XB leg")
+    ap.add_argument("--md-tile", type=int, default=16, help="copies of those regions in the timed NM/MD/XB batch")
     ap.add_argument("--band-model", default=None, help="band model (.npy); default bsbolt_amd/band_model.npy")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")

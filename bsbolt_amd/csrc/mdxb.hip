@@ -4,11 +4,10 @@
 // One lane per task.  A task walks its CIGAR once over the unconverted
 // reference window opac[rb-2, re+2) (bns_get_seq's clamps, bntseq.c:413-437,
 // reversed when rb >= l_pac like the read, bwa.c:264-271) and the read as
-// sequenced, appending to its MD and XB slots.  The window is read straight
-// from the 2-bit opac (four bases per byte, first base in the top bits), so a
-// task touches only the bytes of its window, its read, its CIGAR and its
-// output slots.  The work per task is a few hundred byte operations; the
-// kernel is bound by the latency of those gathers, not by arithmetic.
+// sequenced, appending to its MD and XB slots.  The lane first copies the
+// window (2-bit, 16 bases per dword) and its read into its own LDS rows with
+// independent dword loads, so the walk's byte reads never wait on HBM; a task
+// too long for its rows reads both from HBM instead.
 #include "bsx_internal.h"
 
 namespace bsx {
@@ -17,16 +16,47 @@ __device__ __forceinline__ int pac_at(const uint8_t* __restrict__ pac, int64_t p
   return (pac[p >> 2] >> ((~p & 3) << 1)) & 3;
 }
 
-// appends the decimal digits of v >= 0 (kputw, kstring.h:62-76); false on overflow
-__device__ __forceinline__ bool put_int(char* __restrict__ s, int& l, int cap, int v) {
-  char d[11];
-  int n = 0;
-  if (v == 0) d[n++] = '0';
-  for (; v > 0; v /= 10) d[n++] = (char)('0' + v % 10);
-  if (l + n > cap) return false;
-  while (n > 0) s[l++] = d[--n];
-  return true;
-}
+// A lane's output string: characters gather in a register and leave as one
+// dword store per four (the slot is dword-aligned), or byte by byte otherwise.
+struct StrOut {
+  char* __restrict__ p;
+  int len, cap;
+  uint32_t acc;
+  bool words;
+  __device__ __forceinline__ bool put(int c) {
+    if (len >= cap) return false;
+    acc |= (uint32_t)(c & 0xff) << (8 * (len & 3));
+    ++len;
+    if ((len & 3) == 0) {
+      if (words) {
+        *(uint32_t*)(p + len - 4) = acc;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p[len - 4 + k] = (char)(acc >> (8 * k));
+      }
+      acc = 0;
+    }
+    return true;
+  }
+  __device__ __forceinline__ void flush() {
+    for (int k = 0; k < (len & 3); ++k) p[(len & ~3) + k] = (char)(acc >> (8 * k));
+  }
+  // appends the decimal digits of v >= 0 (kputw, kstring.h:62-76); false on
+  // overflow.  The digits gather in nibbles of one register, most significant
+  // in the low nibble, so no array is indexed at run time.
+  __device__ __forceinline__ bool put_int(int v) {
+    uint64_t dg = 0;
+    int n = 0;
+    do {
+      dg = (dg << 4) | (uint64_t)(v % 10);
+      v /= 10;
+      ++n;
+    } while (v > 0);
+    if (len + n > cap) return false;
+    for (int k = 0; k < n; ++k) put('0' + (int)((dg >> (4 * k)) & 15));
+    return true;
+  }
+};
 
 __device__ __forceinline__ char meth_context(int b1, int b2, int b3, bool meth, int& cg, int& ch) {
   if (b1 == 1 && b2 == 2) { ++cg; return meth ? 'X' : 'x'; }
@@ -35,7 +65,18 @@ __device__ __forceinline__ char meth_context(int b1, int b2, int b3, bool meth, 
   return meth ? 'Z' : 'z';
 }
 
-__global__ void __launch_bounds__(256) md_kernel(MdArgs a) {
+constexpr int kMdBlock = 128;
+constexpr int kMdWinDw = 20;   // LDS dwords per lane for the window: 320 bases
+constexpr int kMdReadDw = 48;  // ... for the read: 192 bytes
+
+__global__ void __launch_bounds__(kMdBlock) md_kernel(MdArgs a) {
+  __shared__ uint32_t s_win[kMdWinDw][kMdBlock];   // column per lane: lanes read distinct banks
+  __shared__ uint32_t s_read[kMdReadDw][kMdBlock];
+  const int lid = threadIdx.x;
+  const uint32_t* __restrict__ opac32 = (const uint32_t*)a.opac;
+  // dword stores for the strings when every slot starts on a dword
+  const bool md_words = ((uintptr_t)a.md & 3) == 0 && (a.md_slot & 3) == 0;
+  const bool xb_words = ((uintptr_t)a.xb & 3) == 0 && (a.xb_slot & 3) == 0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t rb = a.rb[t], re = a.re[t], l_pac = a.l_pac;
     const int l_query = a.l_query[t];
@@ -74,22 +115,53 @@ __global__ void __launch_bounds__(256) md_kernel(MdArgs a) {
     // beg + k; on the reverse strand the reversed reverse-complement window,
     // which is 3 - opac[2*l_pac - end + k]
     const int64_t k0 = fwd ? beg : 2 * l_pac - end;
+    // stage the window and the read in this lane's LDS rows: aligned dwords
+    // that hold at least one byte of the window / read (such a dword lies in
+    // the allocation the byte lies in), all loads issued before any use
+    const int64_t wd0 = k0 >> 4, wd1 = (k0 + clen - 1) >> 4;
+    const uint64_t oq0 = a.oq_off[t];
+    const int64_t rd0 = (int64_t)(oq0 >> 2), rd1 = l_query > 0 ? (int64_t)((oq0 + l_query - 1) >> 2) : rd0;
+    const bool staged = wd1 - wd0 < kMdWinDw && rd1 - rd0 < kMdReadDw;
+    if (staged) {  // every load in flight at once (indices past the last dword repeat it)
+      const uint32_t* __restrict__ rq = (const uint32_t*)a.oquery;
+      const int lw = (int)(wd1 - wd0), lr = (int)(rd1 - rd0);
+      uint32_t wv[kMdWinDw], rv[kMdReadDw];
+#pragma unroll
+      for (int j = 0; j < kMdWinDw; ++j) wv[j] = opac32[wd0 + (j < lw ? j : lw)];
+#pragma unroll
+      for (int j = 0; j < kMdReadDw; ++j) rv[j] = rq[rd0 + (j < lr ? j : lr)];
+#pragma unroll
+      for (int j = 0; j < kMdWinDw; ++j) s_win[j][lid] = wv[j];
+#pragma unroll
+      for (int j = 0; j < kMdReadDw; ++j) s_read[j][lid] = rv[j];
+    }
     bool oob = false;
     auto cs = [&](int64_t k) -> int {
       if (k < 0 || k >= clen) { oob = true; return 4; }
-      const int b = pac_at(a.opac, k0 + k);
+      const int64_t p = k0 + k;
+      int b;
+      if (staged) {
+        const uint32_t w = s_win[(int)((p >> 4) - wd0)][lid];
+        b = (int)((w >> (8 * ((p >> 2) & 3))) >> ((~p & 3) << 1)) & 3;  // byte (p>>2)&3 of the dword
+      } else {
+        b = pac_at(a.opac, p);
+      }
       return fwd ? b : 3 - b;
     };
-    const uint8_t* __restrict__ oq = a.oquery + a.oq_off[t];
-    auto oqb = [&](int x) -> int { return fwd ? oq[x] : oq[l_query - 1 - x]; };
+    const uint8_t* __restrict__ oq = a.oquery + oq0;
+    auto oqb = [&](int x) -> int {
+      const int xx = fwd ? x : l_query - 1 - x;
+      if (!staged) return oq[xx];
+      const uint64_t q = oq0 + (uint64_t)xx;
+      return (int)(s_read[(int)((int64_t)(q >> 2) - rd0)][lid] >> (8 * (q & 3))) & 0xff;
+    };
     const bool reverse = rb > l_pac;  // bwa.c:212 (strictly greater)
     const bool is_crick = (rb >= a.crick_l && rb < l_pac) || rb >= a.crick_l + l_pac;
     const int ref_base = is_crick ? 2 : 1, methyl_base = is_crick ? 0 : 3;
     const char* i2b = fwd ? "ACGTN" : "TGCAN";
-    char* __restrict__ md = a.md + t * a.md_slot;
-    char* __restrict__ xb = a.xb + t * a.xb_slot;
-    const int md_cap = (int)a.md_slot, xb_cap = (int)a.xb_slot;
-    int ml = 0, xl = 0, x = 0, y = 0, u = 0, n_mm = 0, n_gap = 0, meth_pos = 0;
+    StrOut md{a.md + t * a.md_slot, 0, (int)a.md_slot, 0u, md_words};
+    StrOut xb{a.xb + t * a.xb_slot, 0, (int)a.xb_slot, 0u, xb_words};
+    int x = 0, y = 0, u = 0, n_mm = 0, n_gap = 0, meth_pos = 0;
     int cgm = 0, cgu = 0, chm = 0, chu = 0;
     int st = 0;
     for (int k = 0; k < nc && st == 0; ++k) {
@@ -108,19 +180,15 @@ __global__ void __launch_bounds__(256) md_kernel(MdArgs a) {
             if (o == ref_base || o == methyl_base) {
               const char m = o == ref_base ? meth_context(b1, b2, b3, true, cgm, chm)
                                            : meth_context(b1, b2, b3, false, cgu, chu);
-              if (meth_pos > 0 && !put_int(xb, xl, xb_cap, meth_pos)) st = -1;
-              if (st == 0 && xl + 1 > xb_cap) st = -1;
-              if (st == 0) xb[xl++] = m;
+              if ((meth_pos > 0 && !xb.put_int(meth_pos)) || !xb.put(m)) st = -1;
               meth_pos = 0;
               ++u;
             } else {
-              if (!put_int(md, ml, md_cap, u) || ml + 1 > md_cap) st = -1;
-              else md[ml++] = i2b[c];
+              if (!md.put_int(u) || !md.put(i2b[c])) st = -1;
               ++n_mm; ++meth_pos; u = 0;
             }
           } else if (o != c) {
-            if (!put_int(md, ml, md_cap, u) || ml + 1 > md_cap) st = -1;
-            else md[ml++] = i2b[c];
+            if (!md.put_int(u) || !md.put(i2b[c])) st = -1;
             ++n_mm; ++meth_pos; u = 0;
           } else {
             ++u; ++meth_pos;
@@ -130,12 +198,10 @@ __global__ void __launch_bounds__(256) md_kernel(MdArgs a) {
         x += len; y += len;
       } else if (op == 2) {
         if (k > 0 && k < nc - 1) {  // not for a leading or trailing D
-          if (!put_int(md, ml, md_cap, u) || ml + 1 + len > md_cap) st = -1;
-          else {
-            md[ml++] = '^';
-            for (int i = 0; i < len; ++i) md[ml++] = i2b[cs(y + i + 2)];
-            if (oob) st = -2;
-          }
+          if (!md.put_int(u) || !md.put('^')) st = -1;
+          for (int i = 0; i < len && st == 0; ++i)
+            if (!md.put(i2b[cs(y + i + 2)])) st = -1;
+          if (st == 0 && oob) st = -2;
           u = 0; n_gap += len;
         }
         y += len;
@@ -143,22 +209,24 @@ __global__ void __launch_bounds__(256) md_kernel(MdArgs a) {
         x += len; n_gap += len; meth_pos += len;
       }
     }
-    if (st == 0 && !put_int(md, ml, md_cap, u)) st = -1;
-    if (st == 0 && meth_pos > 0 && !put_int(xb, xl, xb_cap, meth_pos)) st = -1;
+    if (st == 0 && !md.put_int(u)) st = -1;
+    if (st == 0 && meth_pos > 0 && !xb.put_int(meth_pos)) st = -1;
     a.status[t] = st;
     if (st != 0) continue;
+    md.flush();
+    xb.flush();
     a.nm[t] = n_mm + n_gap;
-    a.md_len[t] = ml;
-    a.xb_len[t] = xl;
+    a.md_len[t] = md.len;
+    a.xb_len[t] = xb.len;
     meth[0] = cgm; meth[1] = cgu; meth[2] = chm; meth[3] = chu;
   }
 }
 
 hipError_t launch_md(const MdArgs& a, int num_cu, hipStream_t st) {
   if (a.n <= 0) return hipSuccess;
-  int64_t blocks = (a.n + 255) / 256;
-  if (blocks > (int64_t)num_cu * 32) blocks = (int64_t)num_cu * 32;
-  hipLaunchKernelGGL(md_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  int64_t blocks = (a.n + kMdBlock - 1) / kMdBlock;
+  if (blocks > (int64_t)num_cu * 64) blocks = (int64_t)num_cu * 64;
+  hipLaunchKernelGGL(md_kernel, dim3((unsigned)blocks), dim3(kMdBlock), 0, st, a);
   return hipGetLastError();
 }
 

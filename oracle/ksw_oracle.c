@@ -554,3 +554,64 @@ double orc_global2_batch(const int8_t *mat, int o_del, int e_del, int o_ins, int
 	}
 	return now_s() - t0;
 }
+
+/* orc_gen_md over a batch on nthreads host threads (dynamic chunks of 4096
+ * tasks); strings go to fixed slots md[t*md_slot], xb[t*xb_slot].  Returns the
+ * wall-clock seconds. */
+typedef struct {
+	int64_t n, l_pac, crick_l, cigar_slot, md_slot, xb_slot;
+	const uint8_t *opac, *oquery;
+	const int64_t *rb, *re;
+	const uint64_t *oq_off;
+	const int32_t *l_query, *n_cigar;
+	const uint32_t *cigar;
+	char *md, *xb;
+	int32_t *md_len, *xb_len, *nm, *meth, *status;
+	int64_t next;
+	pthread_mutex_t mu;
+} md_job;
+
+static void *md_worker(void *arg)
+{
+	md_job *J = (md_job *)arg;
+	for (;;) {
+		pthread_mutex_lock(&J->mu);
+		const int64_t a = J->next;
+		J->next += 4096;
+		pthread_mutex_unlock(&J->mu);
+		if (a >= J->n) break;
+		const int64_t b = a + 4096 < J->n ? a + 4096 : J->n;
+		for (int64_t t = a; t < b; ++t)
+			J->status[t] = orc_gen_md(J->l_pac, J->opac, J->rb[t], J->re[t], J->crick_l, J->l_query[t],
+			                          J->oquery + J->oq_off[t], J->n_cigar[t], J->cigar + t * J->cigar_slot,
+			                          J->md + t * J->md_slot, (int)J->md_slot, &J->md_len[t],
+			                          J->xb + t * J->xb_slot, (int)J->xb_slot, &J->xb_len[t], &J->nm[t],
+			                          J->meth + 4 * t);
+	}
+	return 0;
+}
+
+double orc_gen_md_batch(int64_t n, int64_t l_pac, const uint8_t *opac, int64_t crick_l, const int64_t *rb,
+                        const int64_t *re, const uint8_t *oquery, const uint64_t *oq_off, const int32_t *l_query,
+                        const int32_t *n_cigar, const uint32_t *cigar, int64_t cigar_slot, char *md, int64_t md_slot,
+                        int32_t *md_len, char *xb, int64_t xb_slot, int32_t *xb_len, int32_t *nm, int32_t *meth,
+                        int32_t *status, int nthreads)
+{
+	md_job J;
+	memset(&J, 0, sizeof J);
+	J.n = n; J.l_pac = l_pac; J.crick_l = crick_l; J.cigar_slot = cigar_slot; J.md_slot = md_slot; J.xb_slot = xb_slot;
+	J.opac = opac; J.oquery = oquery; J.rb = rb; J.re = re; J.oq_off = oq_off; J.l_query = l_query;
+	J.n_cigar = n_cigar; J.cigar = cigar; J.md = md; J.xb = xb; J.md_len = md_len; J.xb_len = xb_len;
+	J.nm = nm; J.meth = meth; J.status = status;
+	pthread_mutex_init(&J.mu, 0);
+	if (nthreads < 1) nthreads = 1;
+	if (nthreads > 256) nthreads = 256;
+	pthread_t th[256];
+	double t0 = now_s();
+	for (int k = 1; k < nthreads; ++k) pthread_create(&th[k], 0, md_worker, &J);
+	md_worker(&J);
+	for (int k = 1; k < nthreads; ++k) pthread_join(th[k], 0);
+	double dt = now_s() - t0;
+	pthread_mutex_destroy(&J.mu);
+	return dt;
+}

@@ -56,6 +56,9 @@ def lib():
         L.orc_bs_convert.argtypes = [C.c_char_p, _I, _I]
         L.orc_gen_md.restype = _I
         L.orc_gen_md.argtypes = [_I64, _P, _I64, _I64, _I64, _I, _P, _I, _P, _P, _I, _P, _P, _I, _P, _P, _P]
+        L.orc_gen_md_batch.restype = C.c_double
+        L.orc_gen_md_batch.argtypes = [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P,
+                                       _I64, _P, _P, _P, _P, _I]
         _lib = L
     return _lib
 
@@ -133,6 +136,28 @@ def gen_md(l_pac, opac, rb, re, crick_l, oquery, cigar, md_cap=4096, xb_cap=4096
     r = lib().orc_gen_md(l_pac, opac.ctypes.data, rb, re, crick_l, len(oq), oq.ctypes.data, len(cg), cg.ctypes.data,
                          md, md_cap, C.byref(ml), xb, xb_cap, C.byref(xl), C.byref(nm), cnt.ctypes.data)
     return r, md.raw[:ml.value].decode(), xb.raw[:xl.value].decode(), nm.value, [int(x) for x in cnt]
+
+
+def gen_md_batch(l_pac, opac, crick_l, rb, re, oquery, oq_off, l_query, n_cigar, cigar, md_slot, xb_slot,
+                 nthreads=1):
+    """orc_gen_md over a batch (cigar [n, slot]) on nthreads host threads.
+    Returns (dict of md / xb (lists of str), nm, meth [n, 4], status; seconds)."""
+    n = len(rb)
+    arr = lambda x, dt: np.ascontiguousarray(x, dt)
+    rb, re, oq_off = arr(rb, np.int64), arr(re, np.int64), arr(oq_off, np.uint64)
+    l_query, n_cigar, cigar = arr(l_query, np.int32), arr(n_cigar, np.int32), arr(cigar, np.uint32).reshape(n, -1)
+    opac, oquery = arr(opac, np.uint8), arr(oquery, np.uint8)
+    md = np.zeros((n, md_slot), np.uint8)
+    xb = np.zeros((n, xb_slot), np.uint8)
+    ml, xl, nm, st = (np.zeros(n, np.int32) for _ in range(4))
+    meth = np.zeros((n, 4), np.int32)
+    dt = lib().orc_gen_md_batch(n, l_pac, opac.ctypes.data, crick_l, rb.ctypes.data, re.ctypes.data,
+                                oquery.ctypes.data, oq_off.ctypes.data, l_query.ctypes.data, n_cigar.ctypes.data,
+                                cigar.ctypes.data, cigar.shape[1], md.ctypes.data, md_slot, ml.ctypes.data,
+                                xb.ctypes.data, xb_slot, xl.ctypes.data, nm.ctypes.data, meth.ctypes.data,
+                                st.ctypes.data, nthreads)
+    return {"md": [bytes(md[i, :ml[i]]).decode() for i in range(n)],
+            "xb": [bytes(xb[i, :xl[i]]).decode() for i in range(n)], "nm": nm, "meth": meth, "status": st}, dt
 
 
 def _u8(a):
