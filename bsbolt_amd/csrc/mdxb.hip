@@ -58,13 +58,6 @@ struct StrOut {
   }
 };
 
-__device__ __forceinline__ char meth_context(int b1, int b2, int b3, bool meth, int& cg, int& ch) {
-  if (b1 == 1 && b2 == 2) { ++cg; return meth ? 'X' : 'x'; }
-  if (b1 == 1 && b3 == 2) { ++ch; return meth ? 'Y' : 'y'; }
-  ++ch;
-  return meth ? 'Z' : 'z';
-}
-
 constexpr int kMdBlock = 128;
 constexpr int kMdWinDw = 20;   // LDS dwords per lane for the window: 320 bases
 constexpr int kMdReadDw = 48;  // ... for the read: 192 bytes
@@ -158,6 +151,12 @@ __global__ void __launch_bounds__(kMdBlock) md_kernel(MdArgs a) {
     const bool reverse = rb > l_pac;  // bwa.c:212 (strictly greater)
     const bool is_crick = (rb >= a.crick_l && rb < l_pac) || rb >= a.crick_l + l_pac;
     const int ref_base = is_crick ? 2 : 1, methyl_base = is_crick ? 0 : 3;
+    // window offsets (from the base's own index) of the three context bases
+    // (bwa.c:290-293): after it on the forward Watson copy, around it on the
+    // others
+    const int o1 = !reverse ? (!is_crick ? 2 : 1) : (!is_crick ? 2 : 3);
+    const int o2 = !reverse ? (!is_crick ? 3 : 2) : (!is_crick ? 1 : 2);
+    const int o3 = !reverse ? (!is_crick ? 4 : 3) : (!is_crick ? 0 : 1);
     const char* i2b = fwd ? "ACGTN" : "TGCAN";
     StrOut md{a.md + t * a.md_slot, 0, (int)a.md_slot, 0u, md_words};
     StrOut xb{a.xb + t * a.xb_slot, 0, (int)a.xb_slot, 0u, xb_words};
@@ -168,31 +167,29 @@ __global__ void __launch_bounds__(kMdBlock) md_kernel(MdArgs a) {
       const uint32_t cw = cig[k];
       const int op = (int)(cw & 0xfu), len = (int)(cw >> 4);
       if (op == 0) {
+        // One shape for every base (the lanes of a wave diverge less): the
+        // context of a C (G on the Crick copy) is read where the reference
+        // reads it, one XB and one MD emission site, counters by select.
         for (int i = 0; i < len && st == 0; ++i) {
           const int c = cs(y + i + 2);
           const int o = oqb(x + i);
-          if (c == ref_base) {
-            int b1, b2, b3;
-            if (!reverse && !is_crick) { b1 = cs(y + i + 2); b2 = cs(y + i + 3); b3 = cs(y + i + 4); }
-            else if (!reverse) { b1 = cs(y + i + 1); b2 = cs(y + i + 2); b3 = cs(y + i + 3); }
-            else if (!is_crick) { b1 = cs(y + i + 2); b2 = cs(y + i + 1); b3 = cs(y + i); }
-            else { b1 = cs(y + i + 3); b2 = cs(y + i + 2); b3 = cs(y + i + 1); }
-            if (o == ref_base || o == methyl_base) {
-              const char m = o == ref_base ? meth_context(b1, b2, b3, true, cgm, chm)
-                                           : meth_context(b1, b2, b3, false, cgu, chu);
+          const bool isref = c == ref_base;
+          const bool ev = isref && (o == ref_base || o == methyl_base);  // methylation call
+          const bool mm = isref ? !ev : o != c;                           // MD mismatch
+          if (isref) {
+            const int b1 = cs(y + i + o1), b2 = cs(y + i + o2), b3 = cs(y + i + o3);
+            if (ev) {
+              const bool meth = o == ref_base;
+              const bool cg = b1 == 1 && b2 == 2, chg = !cg && b1 == 1 && b3 == 2;
+              const char m = (char)((cg ? 'X' : chg ? 'Y' : 'Z') + (meth ? 0 : 32));
+              cgm += meth && cg; chm += meth && !cg; cgu += !meth && cg; chu += !meth && !cg;
               if ((meth_pos > 0 && !xb.put_int(meth_pos)) || !xb.put(m)) st = -1;
-              meth_pos = 0;
-              ++u;
-            } else {
-              if (!md.put_int(u) || !md.put(i2b[c])) st = -1;
-              ++n_mm; ++meth_pos; u = 0;
             }
-          } else if (o != c) {
-            if (!md.put_int(u) || !md.put(i2b[c])) st = -1;
-            ++n_mm; ++meth_pos; u = 0;
-          } else {
-            ++u; ++meth_pos;
           }
+          if (mm && (!md.put_int(u) || !md.put(i2b[c]))) st = -1;
+          n_mm += mm;
+          u = mm ? 0 : u + 1;
+          meth_pos = ev ? 0 : meth_pos + 1;
           if (oob) st = -2;
         }
         x += len; y += len;
