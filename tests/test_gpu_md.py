@@ -127,3 +127,31 @@ def test_md_kernel_on_workload_cigar_pass(eng):
         assert len(idx) > 1000
     finally:
         wl.close()
+
+
+def test_md_kernel_rejects_like_the_oracle(eng):
+    """Inconsistent tasks (CIGAR longer than the read, a soft clip, an empty,
+    reversed, negative or out-of-range region, more ops than the slot holds)
+    get the oracle's status -3 and no strings; a read with N bases and the valid
+    task beside them are computed."""
+    g, pac, opac, l_pac, crick_l = M.index(4000, 3)
+    oq = O.get_seq(l_pac, opac, 100, 150).astype(np.uint8)
+    oqn = oq.copy()
+    oqn[[3, 17, 40]] = 4
+    ok = np.array([50 << 4], np.uint32)
+    cases = [(100, 150, oq, ok), (100, 150, oqn, ok), (100, 150, oq, np.array([51 << 4], np.uint32)),
+             (100, 150, oq, np.array([(2 << 4) | 4, 48 << 4], np.uint32)), (150, 150, oq, ok), (160, 150, oq, ok),
+             (-1, 49, oq, ok), (2 * l_pac - 20, 2 * l_pac + 30, oq, ok)]
+    out = _run(eng, opac, l_pac, crick_l, [c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases],
+               [c[3] for c in cases])
+    for i, (rb, re, q, cg) in enumerate(cases):
+        st, md, xb, nm, meth = O.gen_md(l_pac, opac, rb, re, crick_l, q, cg)
+        assert out["status"][i] == st, i
+        assert (out["md"][i], out["xb"][i], int(out["nm"][i]), list(out["meth"][i])) == (md, xb, nm, meth), i
+    assert list(out["status"]) == [0, 0, -3, -3, -3, -3, -3, -3]
+    # more CIGAR ops than the slot holds
+    cig = np.zeros((1, 1), np.uint32)
+    cig[0, 0] = 50 << 4
+    r = eng.md(opac, l_pac, crick_l, np.array([100], np.int64), np.array([150], np.int64), oq,
+               np.zeros(1, np.uint64), np.array([50], np.int32), np.array([2], np.int32), cig)
+    assert r["status"][0] == -3 and r["nm"][0] == -1

@@ -97,3 +97,31 @@ def test_orc_gen_md_matches_live_reference():
         assert (st, md, xb, nm, meth) == (0, r["md"], r["xb"], r["NM"], r["meth"]), (rb, re)
         seen += 1
     assert seen > 200
+
+
+def test_orc_gen_md_rejects_inconsistent_inputs():
+    """Regions the reference makes no CIGAR for and CIGARs that do not fit the
+    read get status -3 and no strings (the reference would read past its
+    buffers there)."""
+    g, pac, opac, l_pac, crick_l = M.index(4000, 3)
+    oq = O.get_seq(l_pac, opac, 100, 150).astype(np.uint8)
+    ok = np.array([50 << 4], np.uint32)
+    assert O.gen_md(l_pac, opac, 100, 150, crick_l, oq, ok)[0] == 0
+    for rb, re, cig in ((100, 150, np.array([51 << 4], np.uint32)),          # consumes 51 of 50 read bases
+                        (100, 150, np.array([(2 << 4) | 4, 48 << 4], np.uint32)),  # soft clip: not a ksw_global2 op
+                        (150, 150, ok), (160, 150, ok), (-1, 49, ok),         # empty, reversed, negative region
+                        (2 * l_pac - 20, 2 * l_pac + 30, ok)):                 # past the doubled reference
+        st, md, xb, nm, meth = O.gen_md(l_pac, opac, rb, re, crick_l, oq, cig)
+        assert (st, md, xb, nm) == (-3, "", "", -1), (rb, re)
+
+
+def test_reads_with_n_bases():
+    """N (code 4) in the read: a mismatch against the reference base in MD."""
+    g, pac, opac, l_pac, crick_l = M.index(4000, 3)
+    rb, re = 500, 560
+    oq = O.get_seq(l_pac, opac, rb, re).astype(np.uint8)
+    oq[10] = 4
+    st, md, xb, nm, meth = O.gen_md(l_pac, opac, rb, re, crick_l, oq, np.array([60 << 4], np.uint32))
+    assert st == 0 and nm == 1
+    ref = O.get_seq(l_pac, opac, rb, re)
+    assert md == f"10{'ACGT'[ref[10]]}49"
