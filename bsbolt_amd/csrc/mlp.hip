@@ -185,17 +185,23 @@ struct FwdOut {
   f4 logit;
 };
 
-__device__ __forceinline__ void mlp_tile_forward(const MlpDev& M, const bf4 xb[2], FwdOut& o, int lane) {
+// Weights and biases from any address space: w1p/w2p/w3p the BF16 operand
+// copies, b1/b2/b3 the biases (b3 read only for its kBandClasses entries
+// unless B3_PADDED, which promises 16 entries, zero past kBandClasses).
+template <bool B3_PADDED>
+__device__ __forceinline__ void mlp_tile_forward_w(const uint16_t* w1p, const uint16_t* w2p, const uint16_t* w3p,
+                                                   const float* b1, const float* b2, const float* b3, const bf4 xb[2],
+                                                   FwdOut& o, int lane) {
   const int r16 = lane & 15, q4 = (lane >> 4) * 4;
   bf4 h1b[4], h2b[4];
 #pragma unroll
   for (int ob = 0; ob < 4; ++ob) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 2; ++kt) acc = mfma(ld_bf4(M.w1p + (16 * ob + r16) * kFeatPad + 16 * kt + q4), xb[kt], acc);
+    for (int kt = 0; kt < 2; ++kt) acc = mfma(ld_bf4(w1p + (16 * ob + r16) * kFeatPad + 16 * kt + q4), xb[kt], acc);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const float z = acc[v] + M.params[kOffB1 + 16 * ob + q4 + v];
+      const float z = acc[v] + b1[16 * ob + q4 + v];
       acc[v] = z > 0.f ? z : 0.f;
     }
     o.h1[ob] = acc;
@@ -205,10 +211,10 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpDev& M, const bf4 xb[2
   for (int ob = 0; ob < 4; ++ob) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma(ld_bf4(M.w2p + (16 * ob + r16) * 64 + 16 * kt + q4), h1b[kt], acc);
+    for (int kt = 0; kt < 4; ++kt) acc = mfma(ld_bf4(w2p + (16 * ob + r16) * 64 + 16 * kt + q4), h1b[kt], acc);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const float z = acc[v] + M.params[kOffB2 + 16 * ob + q4 + v];
+      const float z = acc[v] + b2[16 * ob + q4 + v];
       acc[v] = z > 0.f ? z : 0.f;
     }
     o.h2[ob] = acc;
@@ -216,10 +222,15 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpDev& M, const bf4 xb[2
   }
   f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kt = 0; kt < 4; ++kt) acc = mfma(ld_bf4(M.w3p + r16 * 64 + 16 * kt + q4), h2b[kt], acc);
+  for (int kt = 0; kt < 4; ++kt) acc = mfma(ld_bf4(w3p + r16 * 64 + 16 * kt + q4), h2b[kt], acc);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) acc[v] += (q4 + v < kBandClasses) ? M.params[kOffB3 + q4 + v] : 0.f;
+  for (int v = 0; v < 4; ++v) acc[v] += B3_PADDED ? b3[q4 + v] : (q4 + v < kBandClasses ? b3[q4 + v] : 0.f);
   o.logit = acc;
+}
+
+__device__ __forceinline__ void mlp_tile_forward(const MlpDev& M, const bf4 xb[2], FwdOut& o, int lane) {
+  mlp_tile_forward_w<false>(M.w1p, M.w2p, M.w3p, M.params + kOffB1, M.params + kOffB2, M.params + kOffB3, xb, o,
+                            lane);
 }
 
 // argmax over the 5 logits of task l%16, valid in lanes 0..15 (ties -> lower class)
@@ -265,10 +276,29 @@ mlp_forward_kernel(MlpDev M, const uint16_t* __restrict__ feats, int64_t n, floa
 
 // Features and forward in one pass for the learned-band dispatch: each lane
 // computes the features of one task into the wave's LDS tile, then the wave
-// runs the four 16-task MLP tiles.
+// runs the four 16-task MLP tiles.  The block copies the BF16 weights and the
+// biases into LDS once, so the tiles read their A operands from LDS.
 __global__ void __launch_bounds__(256)
 band_predict_kernel(ExtArgs A, MlpDev M, uint8_t* __restrict__ cls) {
   __shared__ uint16_t xs[4][64 * kFeatPad];
+  __shared__ __align__(16) uint16_t s_w1[kHidden * kFeatPad], s_w2[kHidden * kHidden], s_w3[16 * kHidden];
+  __shared__ float s_b[2 * kHidden + 16];  // b1, b2, b3 padded with zeros to 16
+  {
+    const uint32_t* g1 = (const uint32_t*)M.w1p;
+    const uint32_t* g2 = (const uint32_t*)M.w2p;
+    const uint32_t* g3 = (const uint32_t*)M.w3p;
+    for (int k = threadIdx.x; k < kHidden * kFeatPad / 2; k += blockDim.x) ((uint32_t*)s_w1)[k] = g1[k];
+    for (int k = threadIdx.x; k < kHidden * kHidden / 2; k += blockDim.x) ((uint32_t*)s_w2)[k] = g2[k];
+    for (int k = threadIdx.x; k < 16 * kHidden / 2; k += blockDim.x) ((uint32_t*)s_w3)[k] = g3[k];
+    for (int k = threadIdx.x; k < 2 * kHidden + 16; k += blockDim.x) {
+      float v = 0.f;
+      if (k < kHidden) v = M.params[kOffB1 + k];
+      else if (k < 2 * kHidden) v = M.params[kOffB2 + k - kHidden];
+      else if (k - 2 * kHidden < kBandClasses) v = M.params[kOffB3 + k - 2 * kHidden];
+      s_b[k] = v;
+    }
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r16 = lane & 15, q4 = (lane >> 4) * 4;
   uint16_t* X = xs[wv];
@@ -300,7 +330,7 @@ band_predict_kernel(ExtArgs A, MlpDev M, uint8_t* __restrict__ cls) {
 #pragma unroll
       for (int kt = 0; kt < 2; ++kt) xb[kt] = ld_bf4(X + (16 * tq + r16) * kFeatPad + 16 * kt + q4);
       FwdOut o;
-      mlp_tile_forward(M, xb, o, lane);
+      mlp_tile_forward_w<true>(s_w1, s_w2, s_w3, s_b, s_b + kHidden, s_b + 2 * kHidden, xb, o, lane);
       const int c = tile_argmax(o.logit, lane);
       const int64_t s = ch * 64 + 16 * tq + r16;
       if (lane < 16 && s < A.n) cls[s] = (uint8_t)c;
