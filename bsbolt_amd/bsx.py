@@ -35,7 +35,7 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
-           "bsx_set_side_streams", "bsx_md_batch_device")
+           "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -145,6 +145,8 @@ def lib():
         L.bsx_band_model_get.argtypes = [P, P, C.c_int64]
         L.bsx_band_features_device.argtypes = [P, C.POINTER(Scoring), C.POINTER(_ExtTasks), P]
         L.bsx_band_labels_device.argtypes = [P, C.POINTER(Scoring), C.POINTER(_ExtTasks), P]
+        L.bsx_band_predict_device.argtypes = [P, C.POINTER(Scoring), C.POINTER(_ExtTasks), P]
+        L.bsx_band_predict_device.restype = C.c_int
         L.bsx_band_forward_device.argtypes = [P, P, C.c_int64, P, P]
         L.bsx_band_grad_device.argtypes = [P, P, P, C.c_int64, P]
         L.bsx_band_train_device.argtypes = [P, P, P, C.c_int64, C.c_int64, C.c_int, C.c_float, P]
@@ -421,6 +423,16 @@ class Engine:
         self.band_labels_device(dev, lab.ptr, scoring)
         self.sync()
         return lab.numpy()
+
+    def band_predict(self, tasks: ExtTasks, scoring: Scoring = None):
+        """Classes the learned policy's fused predictor gives a host batch."""
+        sc = scoring or Scoring.make()
+        keep, dev = self.tasks_to_device(tasks)
+        dev["n"] = len(tasks)
+        cl = self.empty((len(tasks),), np.uint8)
+        self._check(self.L.bsx_band_predict_device(self.ctx, C.byref(sc), C.byref(self._ext_tasks_dev(dev)), cl.ptr))
+        self.sync()
+        return cl.numpy()
 
     def band_forward(self, feats):
         """MLP forward of host features uint16 [n, 32] -> (logits [n, 5], class [n])."""

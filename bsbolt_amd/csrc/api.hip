@@ -492,6 +492,17 @@ int bsx_band_forward_device(bsx_ctx* c, const uint16_t* feats, int64_t n, float*
   return BSX_OK;
 }
 
+int bsx_band_predict_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, uint8_t* cls) {
+  if (!c || !t || (t->n > 0 && !cls)) return fail(c, BSX_EINVAL, "NULL argument");
+  if (!c->model_ready) return fail(c, BSX_EINVAL, "no band model");
+  ExtArgs a;
+  int rc = ext_args(c, sc, t, a);
+  if (rc) return rc;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), cls, c->num_cu, c->stream), "band predict");
+  return BSX_OK;
+}
+
 static int band_grad(bsx_ctx* c, const uint16_t* feats, const uint8_t* labels, int64_t n, float* grad) {
   HIPCHK(c, c->part.ensure(4 * (size_t)(kParamCount + 1) * (size_t)mlp_grad_waves(n)), "hipMalloc(part)");
   HIPCHK(c, mlp_launch_grad(model_view(c), feats, labels, n, (float*)c->part.p, grad, c->stream), "band grad");

@@ -236,6 +236,10 @@ int bsx_band_labels_device(bsx_ctx *ctx, const bsx_scoring *sc, const bsx_ext_ta
                            uint8_t *labels);
 int bsx_band_forward_device(bsx_ctx *ctx, const uint16_t *feats, int64_t n, float *logits,
                             uint8_t *cls);
+/* The fused features + forward pass the learned policy runs before routing:
+ * cls[n] = the predicted class of each task (device memory). */
+int bsx_band_predict_device(bsx_ctx *ctx, const bsx_scoring *sc, const bsx_ext_tasks *tasks,
+                            uint8_t *cls);
 int bsx_band_grad_device(bsx_ctx *ctx, const uint16_t *feats, const uint8_t *labels, int64_t n,
                          float *grad);
 int bsx_band_train_device(bsx_ctx *ctx, const uint16_t *feats, const uint8_t *labels, int64_t n,

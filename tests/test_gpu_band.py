@@ -191,6 +191,21 @@ def test_adam_step_matches_oracle(engine):
     assert np.abs(p1 - want).max() < 1e-6
 
 
+def test_fused_predictor_matches_features_and_forward(engine, wl_small):
+    """The learned policy's fused pass (features into LDS, MFMA tiles with the
+    weights staged in LDS) gives the classes of the separate features and
+    forward kernels, task for task (the same BF16 operands and MFMA order)."""
+    b = _both(wl_small)
+    p = B.init_params(5)
+    p[-5:] = [0.4, 0.1, -0.2, 0.3, 0.0]
+    engine.set_band_model(p)
+    X = engine.band_features(b)
+    _, cl = engine.band_forward(X)
+    fused = engine.band_predict(b)
+    assert np.array_equal(fused, cl)
+    assert len(set(fused.tolist())) > 1
+
+
 def test_learned_policy_results_equal_plain(engine, wl_small):
     from bsbolt_amd import bsx
     b = _both(wl_small)
