@@ -163,21 +163,25 @@ __global__ void __launch_bounds__(kMdBlock) md_kernel(MdArgs a) {
     int x = 0, y = 0, u = 0, n_mm = 0, n_gap = 0, meth_pos = 0;
     int cgm = 0, cgu = 0, chm = 0, chu = 0;
     int st = 0;
-    for (int k = 0; k < nc && st == 0; ++k) {
-      const uint32_t cw = cig[k];
+    // One step is one base of an M op or one whole I/D op, so a wave's lanes
+    // stay in step across their different CIGAR layouts.  Each base has one
+    // shape (the lanes diverge less): the context of a C (G on the Crick copy)
+    // is read where the reference reads it, one XB and one MD emission site,
+    // counters by select.
+    int k = 0, pos = 0;  // op index, bases of the current M op walked
+    uint32_t cw = nc > 0 ? cig[0] : 0u;
+    while (k < nc && st == 0) {
       const int op = (int)(cw & 0xfu), len = (int)(cw >> 4);
+      bool next = true;
       if (op == 0) {
-        // One shape for every base (the lanes of a wave diverge less): the
-        // context of a C (G on the Crick copy) is read where the reference
-        // reads it, one XB and one MD emission site, counters by select.
-        for (int i = 0; i < len && st == 0; ++i) {
-          const int c = cs(y + i + 2);
-          const int o = oqb(x + i);
+        if (pos < len) {
+          const int c = cs(y + pos + 2);
+          const int o = oqb(x + pos);
           const bool isref = c == ref_base;
           const bool ev = isref && (o == ref_base || o == methyl_base);  // methylation call
           const bool mm = isref ? !ev : o != c;                           // MD mismatch
           if (isref) {
-            const int b1 = cs(y + i + o1), b2 = cs(y + i + o2), b3 = cs(y + i + o3);
+            const int b1 = cs(y + pos + o1), b2 = cs(y + pos + o2), b3 = cs(y + pos + o3);
             if (ev) {
               const bool meth = o == ref_base;
               const bool cg = b1 == 1 && b2 == 2, chg = !cg && b1 == 1 && b3 == 2;
@@ -191,8 +195,10 @@ __global__ void __launch_bounds__(kMdBlock) md_kernel(MdArgs a) {
           u = mm ? 0 : u + 1;
           meth_pos = ev ? 0 : meth_pos + 1;
           if (oob) st = -2;
+          ++pos;
+          next = pos >= len;
         }
-        x += len; y += len;
+        if (next) { x += len; y += len; }
       } else if (op == 2) {
         if (k > 0 && k < nc - 1) {  // not for a leading or trailing D
           if (!md.put_int(u) || !md.put('^')) st = -1;
@@ -204,6 +210,11 @@ __global__ void __launch_bounds__(kMdBlock) md_kernel(MdArgs a) {
         y += len;
       } else if (op == 1) {
         x += len; n_gap += len; meth_pos += len;
+      }
+      if (next) {
+        ++k;
+        pos = 0;
+        cw = k < nc ? cig[k] : 0u;
       }
     }
     if (st == 0 && !md.put_int(u)) st = -1;
