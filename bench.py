@@ -125,6 +125,56 @@ def learned_leg(args, eng, ptrs, outs, cells_d, plain_out, plain_cells, n, total
     return res
 
 
+def cigar_leg(args, eng, gt):
+    """Times ksw_global2 with CIGARs (bsx_global_batch_device) on every try of a
+    CIGAR pass (mem_reg2aln's band inference and retries), as one resident batch
+    of 4*md_tile copies; the oracle's restatement on one host thread is the
+    baseline and the parity check on the distinct tasks."""
+    from bsbolt_amd.tasks import GlbTasks
+    m = len(gt)
+    reps = max(1, 4 * args.md_tile)  # few regions need DP, so more copies fill the GPU
+    qtot, ttot = len(gt.query), len(gt.target)
+    t = GlbTasks(np.tile(gt.query, reps), np.concatenate([gt.q_off + r * qtot for r in range(reps)]),
+                 np.tile(gt.qlen, reps), np.tile(gt.target, reps),
+                 np.concatenate([gt.t_off + r * ttot for r in range(reps)]), np.tile(gt.tlen, reps),
+                 np.tile(gt.w, reps))
+    n = len(t)
+    slot = int((gt.qlen + gt.tlen).max())
+    keep = {k: eng.to_device(np.ascontiguousarray(getattr(t, k))) for k in
+            ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")}
+    res = {"score": eng.empty((n,), np.int32), "n_cigar": eng.empty((n,), np.int32),
+           "cigar": eng.empty((n, slot), np.uint32)}
+    dev = {k: v.ptr for k, v in {**keep, **res}.items()}
+    dev.update(n=n, cigar_slot=slot)
+    for _ in range(max(1, args.warmup)):
+        eng.global_device(dev)
+    eng.sync()
+    eng.timer_start()
+    for _ in range(args.steps):
+        eng.global_device(dev)
+    ms = eng.timer_stop() / args.steps
+    # banded cells of ksw_global2 (ksw.c:530-560): rows x min(qlen, 2w+1)
+    band = np.minimum(gt.qlen.astype(np.int64), 2 * gt.w.astype(np.int64) + 1)
+    cells = int((band * gt.tlen.astype(np.int64)).sum()) * reps
+    log(f"CIGAR pass (ksw_global2): {n} tasks ({m} distinct x {reps}), {ms:.3f} ms")
+    entry = {"value": n / (ms / 1e3), "unit": "alignments/s", "tasks": n, "ms_per_pass": ms,
+             "gcups_banded": cells / (ms / 1e3) / 1e9, "cpu_baseline": None}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        sc_, nc_, cg_, dt = O.global2_batch(gt, slot)
+        got_s, got_n = res["score"].numpy()[:m], res["n_cigar"].numpy()[:m]
+        got_c = np.empty((m, slot), np.uint32)  # the distinct tasks' CIGAR slots only
+        eng._check(eng.L.bsx_memcpy(eng.ctx, got_c.ctypes.data, res["cigar"].ptr, got_c.nbytes, 2))
+        mism = int((got_s != sc_).sum() + (got_n != nc_).sum())
+        mism += sum(not np.array_equal(got_c[i, :nc_[i]], cg_[i, :nc_[i]]) for i in range(m) if nc_[i] > 0)
+        entry["cpu_baseline"] = {"value": m / dt, "unit": "alignments/s", "cores": 1, "kind": "port",
+                                 "sample": f"the {m} distinct tasks, oracle restatement of ksw.c:486-606",
+                                 "seconds": dt, "mismatches_vs_gpu": int(mism)}
+        log(f"CIGAR pass CPU restatement: {m / dt / 1e6:.3f} M tasks/s on 1 thread, {mism} mismatches vs GPU")
+    del keep, res
+    return entry
+
+
 def md_leg(args, eng):
     """Times the NM / MD:Z / XB:Z pass (the tail of bwa_gen_cigar2, bwa.c:250-343)
     on the GPU: the final CIGAR of every alignment region of the same workload
@@ -139,6 +189,7 @@ def md_leg(args, eng):
         opac, l_pac, crick_l = wl.opac(), wl.l_pac, cfg.genome_len
     finally:
         wl.close()
+    glb = cigar_leg(args, eng, out[0])
     # the regions' tasks repeated md_tile times: a batch the size of a full
     # chunk of reads, so the pass is timed at full occupancy
     reps = max(1, args.md_tile)
@@ -187,6 +238,7 @@ def md_leg(args, eng):
             f"{mism} mismatches vs GPU")
     del keep, outs
     return {"value": n / (ms / 1e3), "unit": "alignments/s", "regions": n, "ms_per_pass": ms, "cpu_baseline": cpu,
+            "cigar_pass": glb,
             "workload": f"{args.workload} x {args.md_scale}: final CIGAR of every alignment region "
                         f"({len(idx)} regions, batch of {reps} copies)",
             "status_ok": int((status == 0).sum()), "status_bridging": int((status == 1).sum()),

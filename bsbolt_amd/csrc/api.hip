@@ -669,7 +669,7 @@ int bsx_timer_stop(bsx_ctx* c, float* ms) {
 }
 
 void bsx_set_global_zslot(bsx_ctx* c, int64_t bytes) {
-  if (c && bytes >= 64) c->zslot = bytes;
+  if (c && bytes >= 64) c->zslot = bytes & ~(int64_t)3;  // whole dwords: the backtrack reads dword-wise
 }
 
 int bsx_md_batch_device(bsx_ctx* c, const bsx_md_tasks* t, bsx_md_results* r) {

@@ -1,9 +1,10 @@
 // Bit-exact batched ksw_global2 (bsbolt/External/BWA/ksw.c:504-606) for gfx950.
 //
-// Same layout as the extension kernel: a group of L lanes owns one task, lane g
-// holds query columns [g*C, g*C+C) of H(i-1,j-1) and E(i,j) in VGPRs, rows are
-// processed in order.  The band is fixed (beg = max(i-w,0), end = min(i+w+1,qlen),
-// ksw.c:530-531).  F along a row is
+// A group of L lanes owns one task and holds the band of the current row: lane
+// g keeps positions r = j - beg in [g*C, g*C+C) of H(i-1,j-1) and E(i,j) in
+// VGPRs, so the group's width follows n_col = min(qlen, 2w+1), not qlen; rows
+// are processed in order.  The band is fixed (beg = max(i-w,0),
+// end = min(i+w+1,qlen), ksw.c:530-531).  F along a row is
 //   F(j) = max_{beg<=k<j}(M_k - oe_ins + k*e_ins) - (j-1)*e_ins   (j > beg),
 //   F(beg) = -inf,
 // which equals the reference's sequential recurrence whenever every in-band M
@@ -13,31 +14,67 @@
 // The direction byte of every in-band cell (bits 0-1: source of H, bit 2: E
 // extended, bit 5: F extended; ksw.c:551-564) is written to the group's slot in
 // a scratch buffer, row-major with n_col = min(qlen, 2w+1) bytes per row, and
-// lane 0 of the group walks it back (ksw.c:588-602), writing the CIGAR into the
-// task's caller-sized slot.
+// the group walks it back (ksw.c:588-602), writing the CIGAR into the task's
+// caller-sized slot.  The walk reads the slot through an LDS window of whole
+// rows: the group loads the next rows with all its lanes at once (one HBM
+// latency per window, not one per step) and every step then reads LDS.
 #include "bsx_internal.h"
 
 namespace bsx {
 
 constexpr int kNegInf = -0x40000000;  // MINUS_INF, ksw.c:490
 constexpr int kGlbMaxT = 1 << 16;
+constexpr int kZWinDw = 256;  // dwords of a group's LDS window on its direction bytes
 
+// Group scans and reductions on DPP (L = 16, 32 or 64 lanes; values may be
+// negative, so lanes without a source keep the identity instead of 0).
+template <int L>
+__device__ __forceinline__ int g_incl_max(int x, int ident) {
+  x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  if (L >= 32) x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  if (L >= 64) x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return x;
+}
+
+// exclusive prefix max over the group (ident for its first lane)
 template <int L>
 __device__ __forceinline__ int g_excl_max(int x, int lig, int ident) {
-#pragma unroll
-  for (int d = 1; d < L; d <<= 1) {
-    int y = __shfl_up(x, d, L);
-    if (lig >= d) x = max(x, y);
-  }
-  int e = __shfl_up(x, 1, L);
+  const int incl = g_incl_max<L>(x, ident);
+  const int e = L == 16 ? __builtin_amdgcn_update_dpp(ident, incl, 0x111, 0xf, 0xf, false)   // row_shr:1
+                        : __builtin_amdgcn_update_dpp(ident, incl, 0x138, 0xf, 0xf, false);  // wave_shr:1
   return lig == 0 ? ident : e;
 }
 
+// every lane of the group receives the maximum
 template <int L>
 __device__ __forceinline__ int g_max(int x) {
-#pragma unroll
-  for (int d = L / 2; d >= 1; d >>= 1) x = max(x, __shfl_xor(x, d, L));
-  return x;
+  if (L == 16) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true));   // quad_perm [1,0,3,2]
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true));   // quad_perm [2,3,0,1]
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true));  // row_half_mirror
+    return max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true));  // row_mirror
+  }
+  x = g_incl_max<L>(x, INT32_MIN);
+  if (L == 64) return __builtin_amdgcn_readlane(x, 63);
+  const int a = __builtin_amdgcn_readlane(x, 31), b = __builtin_amdgcn_readlane(x, 63);
+  return (int)(threadIdx.x & 63) < 32 ? a : b;
+}
+
+// value of the previous lane of the group (lane 0: unused by the callers)
+template <int L>
+__device__ __forceinline__ int g_prev(int x) {
+  return L == 16 ? __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true)   // row_shr:1
+                 : __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true);  // wave_shr:1
+}
+
+// value of the next lane of the group (lane L-1: unused by the callers)
+template <int L>
+__device__ __forceinline__ int g_next(int x) {
+  return L == 16 ? __builtin_amdgcn_update_dpp(0, x, 0x101, 0xf, 0xf, true)   // row_shl:1
+                 : __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true);  // wave_shl:1
 }
 
 template <int L, int C>
@@ -52,6 +89,8 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
   const int32_t* __restrict__ list = W.list + lbase;
   const uint32_t nlist = W.counts[bin];
   uint8_t* __restrict__ z = W.z + (int64_t)group * W.zslot;
+  __shared__ uint32_t zwin_all[256 / L][kZWinDw];
+  uint32_t* __restrict__ zwin = zwin_all[threadIdx.x / L];
   const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
   const int e_del = A.e_del, e_ins = A.e_ins;
 
@@ -62,17 +101,28 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
     const uint32_t idx = __shfl(mine, 0, L);
     if (idx >= nlist) break;
     const int task = list[idx];
-    const int qlen = A.qlen[task], tlen = A.tlen[task], w = A.w[task];
+    const int qlen = A.qlen[task], tlen = A.tlen[task];
+    // a band wider than both sequences is the same band (beg = 0, end = qlen
+    // on every row): clamping keeps i + w + 1 and the like in int range
+    const int w = min(A.w[task], max(qlen, tlen));
     const uint64_t qoff = A.q_off[task], toff = A.t_off[task];
     const int ncol = qlen < 2 * w + 1 ? qlen : 2 * w + 1;
     const bool want_cigar = A.cigar != nullptr;
 
+    // Lanes hold band positions r = j - beg (lane g: r in [g*C, g*C+C)), so a
+    // group needs L*C >= n_col columns whatever qlen is.  Row i works on
+    // positions [0, end-beg).  While beg stays 0 (i < w) the state moves one
+    // position right each row (H(i,j-1) becomes the next row's H(i-1,j-1));
+    // once beg advances with i, position r of row i+1 is column beg+1+r, so
+    // H stays in place and E and the query profile move one position left, the
+    // top position taking query[beg+1+L*C-1].
+    constexpr int LC = L * C;
     int G[C], E[C];
     uint32_t plo[C];
     int phi[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int j = base + c;
+      const int j = base + c;  // row 0: beg = 0
       const int qc = j < qlen ? (int)A.query[qoff + j] : 4;
       uint32_t lo = A.prof_lo[4];
       int hi = A.prof_hi[4];
@@ -88,35 +138,49 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
     }
     int hq = qlen <= w ? -(A.o_ins + e_ins * qlen) : kNegInf;  // eh[qlen].h
 
-    uint32_t tch = 0;
-    int tcb = -(1 << 30);
-    for (int i = 0; i < tlen; ++i) {
-      if (i - tcb >= 4 * L) {
-        tcb = i;
-        uint32_t v = 0;
+    // Per-row codes, two rows per lane and 2L rows per chunk: byte 0 the
+    // target base of row rho, byte 1 the query base entering the top position
+    // after row rho (column rho + L*C - w; 4 outside the query).  The next
+    // chunk is loaded one chunk ahead.
+    auto chunk = [&](int cb) -> uint32_t {
+      uint32_t v = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int r = i + 4 * lig + k;
-          v |= (r < tlen ? (uint32_t)A.target[toff + r] : 0u) << (8 * k);
-        }
-        tch = v;
+      for (int k = 0; k < 2; ++k) {
+        const int rho = cb + 2 * lig + k;
+        const int jq = rho + LC - w;
+        const uint32_t tb = rho < tlen ? (uint32_t)A.target[toff + rho] : 0u;
+        const uint32_t qb = (jq >= 0 && jq < qlen) ? (uint32_t)A.query[qoff + jq] : 4u;
+        v |= (tb | qb << 8) << (16 * k);
       }
-      const int tr = i - tcb;
-      const int t = (int)((__shfl(tch, tr >> 2, L) >> (8 * (tr & 3))) & 0xffu);
+      return v;
+    };
+    uint32_t ch = chunk(0), nx = chunk(2 * L);
+    int cb = 0;
+    for (int i = 0; i < tlen; ++i) {
+      int tr = i - cb;
+      if (tr >= 2 * L) {
+        cb = i;
+        tr = 0;
+        ch = nx;
+        nx = chunk(i + 2 * L);
+      }
+      const uint32_t wd = (uint32_t)__shfl((int)ch, tr >> 1, L) >> (16 * (tr & 1));
+      const int t = (int)(wd & 0xffu), qn = (int)((wd >> 8) & 0xffu);
       const int beg = i > w ? i - w : 0;
       const int end = i + w + 1 < qlen ? i + w + 1 : qlen;
+      const int nb = end - beg;  // in-band positions [0, nb)
       const int h1p = beg == 0 ? -(A.o_del + e_del * (i + 1)) : kNegInf;
 
       int M[C], ex[C];
       int run = INT32_MIN;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const int j = base + c;
-        const bool inb = (j >= beg) & (j < end);
+        const int r = base + c;
+        const bool inb = r < nb;
         const int s = t < 4 ? (int)((int32_t)(plo[c] << (24 - 8 * t)) >> 24) : phi[c];
         const int mm = G[c] + s;
         M[c] = mm;
-        const int key = inb ? mm - oe_ins + j * e_ins : INT32_MIN;
+        const int key = inb ? mm - oe_ins + r * e_ins : INT32_MIN;
         ex[c] = run;
         run = max(run, key);
       }
@@ -125,12 +189,12 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
       int h1c = INT32_MIN;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const int j = base + c;
-        const bool inb = (j >= beg) & (j < end);
+        const int r = base + c;
+        const bool inb = r < nb;
         const int x = max(ex[c], lex);
-        // columns without an in-band predecessor get -inf (this also keeps the
-        // subtraction away from INT32_MIN on out-of-band columns)
-        const int f = (j == beg || x == INT32_MIN) ? kNegInf : x - (j - 1) * e_ins;
+        // positions without an in-band predecessor get -inf (this also keeps
+        // the subtraction away from INT32_MIN on out-of-band positions)
+        const int f = (r == 0 || x == INT32_MIN) ? kNegInf : x - (r - 1) * e_ins;
         const int mm = M[c], e = E[c];
         int d = mm >= e ? 0 : 1;
         int hh = mm >= e ? mm : e;
@@ -141,23 +205,45 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
         const int ne = ee > od ? ee : od;
         const int oi = mm - oe_ins, ff = f - e_ins;
         if (ff > oi) d |= 2 << 4;
-        if (inb) {
-          E[c] = ne;
-          if (want_cigar) z[(int64_t)i * ncol + (j - beg)] = (uint8_t)d;
-          if (j == end - 1) h1c = hh;
+        // out-of-band positions hold E = -inf: the position end (eh[end].e,
+        // ksw.c:575) and every position that enters the band later
+        E[c] = inb ? ne : kNegInf;
+        if (want_cigar && inb) z[(int64_t)i * ncol + r] = (uint8_t)d;
+        if (r == nb - 1) h1c = hh;
+      }
+      if (i >= w) {  // beg(i+1) = beg(i) + 1
+#pragma unroll
+        for (int c = 0; c < C; ++c) G[c] = h[c];
+        const int e0 = g_next<L>(E[0]);
+        const uint32_t p0 = (uint32_t)g_next<L>((int)plo[0]);
+        const int q0 = g_next<L>(phi[0]);
+#pragma unroll
+        for (int c = 0; c + 1 < C; ++c) {
+          E[c] = E[c + 1];
+          plo[c] = plo[c + 1];
+          phi[c] = phi[c + 1];
+        }
+        uint32_t lo = A.prof_lo[4];
+        int hi = A.prof_hi[4];
+        if (qn == 0) { lo = A.prof_lo[0]; hi = A.prof_hi[0]; }
+        if (qn == 1) { lo = A.prof_lo[1]; hi = A.prof_hi[1]; }
+        if (qn == 2) { lo = A.prof_lo[2]; hi = A.prof_hi[2]; }
+        if (qn == 3) { lo = A.prof_lo[3]; hi = A.prof_hi[3]; }
+        const bool top = lig == L - 1;
+        E[C - 1] = top ? kNegInf : e0;
+        plo[C - 1] = top ? lo : p0;
+        phi[C - 1] = top ? hi : q0;
+      } else {  // beg stays 0: H(i,j-1) moves to position j, eh[0].h = h1p
+        const int hprev = g_prev<L>(h[C - 1]);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const int r = base + c;
+          const int left = c == 0 ? hprev : h[c - 1];
+          G[c] = r == 0 ? h1p : left;
         }
       }
-      const int hprev = __shfl_up(h[C - 1], 1, L);
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const int j = base + c;
-        const int left = c == 0 ? hprev : h[c - 1];
-        if ((j >= beg) & (j <= end) & (j < qlen)) G[c] = j == beg ? h1p : left;
-        if (j == end && j < qlen) E[c] = kNegInf;
-      }
       h1c = g_max<L>(h1c);
-      const int h1 = end > beg ? h1c : h1p;
-      if (end == qlen) hq = h1;  // eh[end] = {h1, -inf} with end == qlen
+      if (end == qlen) hq = h1c;  // eh[end] = {h1, -inf} with end == qlen
     }
     if (lig == 0) {
       if (A.score) A.score[task] = hq;
@@ -194,14 +280,37 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
       };
       int i = tlen - 1;
       int k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
+      // LDS window: rows [wlo, whi] of the slot, from byte wbase (a dword
+      // boundary) on; rows per window so that they fit with the alignment slack
+      const int wrows = (4 * kZWinDw - 3) / ncol;
+      const uint32_t* __restrict__ zw = (const uint32_t*)z;
+      int wlo = 1, whi = 0;
+      int64_t wbase = 0;
       while (i >= 0 && k >= 0) {
         const int lo = i > w ? i - w : 0;
         if (k < lo || k - lo >= ncol) {  // cannot happen under the preconditions; never read outside the slot
           over = true;
           break;
         }
-        // lane 0's byte, broadcast: every lane takes the same step by construction
-        const int zb = __shfl((int)z[(int64_t)i * ncol + (k - lo)], 0, L);
+        const int64_t zi = (int64_t)i * ncol + (k - lo);
+        int zb;
+        if (wrows >= 1) {
+          if (i < wlo || i > whi) {  // refill: rows [i - wrows + 1, i], every lane loading
+            const int r0 = i - wrows + 1 > 0 ? i - wrows + 1 : 0;
+            const int64_t d0 = ((int64_t)r0 * ncol) >> 2, d1 = ((int64_t)(i + 1) * ncol - 1) >> 2;
+            __builtin_amdgcn_wave_barrier();
+            for (int q = lig; q <= (int)(d1 - d0); q += L) zwin[q] = zw[d0 + q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wlo = r0; whi = i; wbase = d0 << 2;
+          }
+          const int64_t q = zi - wbase;  // every lane reads the same byte
+          zb = (int)(zwin[q >> 2] >> (8 * (q & 3))) & 0xff;
+        } else {
+          // lane 0's byte, broadcast: every lane takes the same step by construction
+          zb = __shfl((int)z[zi], 0, L);
+        }
         state = zb >> (state << 1) & 3;
         if (state == 0) { push(0, 1); --i; --k; }
         else if (state == 1) { push(2, 1); --i; }
@@ -230,10 +339,10 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
   }
 }
 
-__device__ __forceinline__ int glb_bin_of(int qlen) {
+__device__ __forceinline__ int glb_bin_of(int ncol) {
 #pragma unroll
   for (int b = 0; b < kGlbBins; ++b)
-    if (qlen <= glb_bin_limit(b)) return b;
+    if (ncol <= glb_bin_limit(b)) return b;
   return -1;
 }
 
@@ -245,10 +354,10 @@ glb_bin_count_kernel(GlbArgs A, GlbWork W) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int ql = A.qlen[t], tl = A.tlen[t], w = A.w[t];
-    int b = glb_bin_of(ql);
     const int diff = tl > ql ? tl - ql : ql - tl;
     const int64_t ncol = ql < 2 * (int64_t)w + 1 ? ql : 2 * (int64_t)w + 1;
-    const bool ok = ql >= 1 && tl >= 1 && tl <= kGlbMaxT && w >= 0 && b >= 0 && diff <= w &&
+    int b = glb_bin_of((int)ncol);  // by band width: a group holds n_col positions
+    const bool ok = ql >= 1 && ql <= kGlbMaxT && tl >= 1 && tl <= kGlbMaxT && w >= 0 && b >= 0 && diff <= w &&
                     (A.cigar == nullptr || ncol * tl <= W.zslot);
     if (!ok) {
       b = 0xff;

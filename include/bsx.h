@@ -81,8 +81,10 @@ typedef struct {
 	int64_t *cells;   /* DP cells evaluated: sum over rows of (end - beg) */
 } bsx_ext_results;
 
-/* One ksw_global2 call per task.  Preconditions: qlen >= 1, tlen >= 1,
- * |tlen - qlen| <= w (the reference reads uninitialised memory otherwise). */
+/* One ksw_global2 call per task.  Preconditions: 1 <= qlen, tlen <= 65536,
+ * min(qlen, 2w+1) <= BSX_MAX_QLEN (the band width, which sets the lanes per
+ * task), |tlen - qlen| <= w (the reference reads uninitialised memory
+ * otherwise); a task outside them fails the call with BSX_EINVAL. */
 typedef struct {
 	int64_t n;
 	const uint8_t  *query;

@@ -77,3 +77,39 @@ def test_global_cigar_slot_overflow(engine):
     tk = GlbTasks.from_pairs([(q, t)], 10)
     (score, ncig, cig), rc = engine.global_(tk, slot=2, raise_on_error=False)
     assert rc == BSX_ERANGE and ncig[0] == -1
+
+
+def test_global_long_queries_narrow_and_wide_bands(engine):
+    """Queries longer than any group (the lanes hold the band, not the query),
+    w = 0, and bands wider than both sequences (taken as the full matrix)."""
+    from bsbolt_amd.tasks import GlbTasks
+    rng = np.random.default_rng(45)
+    pairs, ws = [], []
+    for k in range(300):
+        ql = int(rng.integers(2100, 5000)) if k < 200 else int(rng.integers(1, 300))
+        q = rng.integers(0, 5, ql).astype(np.uint8)
+        t = list(np.where(rng.random(ql) < 0.05, rng.integers(0, 4, ql), q % 4))
+        if k % 3:
+            for _ in range(int(rng.integers(1, 4))):
+                p = int(rng.integers(0, len(t) + 1)); L = int(rng.integers(1, 6))
+                if rng.random() < 0.5:
+                    t[p:p] = list(rng.integers(0, 4, L))
+                else:
+                    del t[p:p + L]
+        t = np.array(t or [1], np.uint8)
+        d = abs(len(t) - ql)
+        w = (d + int(rng.integers(0, 40))) if k < 200 else [d, 10**6, 2**28][k % 3]
+        pairs.append((q, t)); ws.append(w)
+    tk = GlbTasks.from_pairs(pairs, ws)
+    slot = int((tk.qlen + tk.tlen).max())
+    engine.set_global_zslot(int((np.minimum(tk.qlen.astype(np.int64), 2 * tk.w.astype(np.int64) + 1)
+                                 * tk.tlen).max()) + 64)
+    try:
+        score, ncig, cig = engine.global_(tk, slot=slot)
+    finally:
+        engine.set_global_zslot(1 << 16)
+    os_, on, oc, _ = O.global2_batch(tk, slot)
+    assert np.array_equal(score, os_)
+    assert np.array_equal(ncig, on)
+    for i in range(len(tk)):
+        assert np.array_equal(cig[i, :ncig[i]], oc[i, :on[i]]), f"task {i}"
