@@ -239,7 +239,7 @@ struct MdArgs {
 hipError_t launch_md(const MdArgs& a, int num_cu, hipStream_t st);
 
 // Blocks per CU of the global kernel; the scratch holds one slot per group.
-constexpr int kGlbBlocksPerCU = 4;
+constexpr int kGlbBlocksPerCU = 7;
 int64_t glb_groups(int num_cu);
 hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st);
 

@@ -94,9 +94,15 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
   const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
   const int e_del = A.e_del, e_ins = A.e_ins;
 
+  // An empty bin returns before its first fetch (uniform over the grid): each
+  // wave's fetch is a device atomic on one address, ~10 ns apiece serialised.
+  if (nlist == 0) return;
   for (;;) {
     // Branch-free fetch: every lane issues the atomic (only lane 0 of the group
-    // adds 1), so no lane-divergent region precedes the broadcast.
+    // adds 1; the compiler folds it into one atomic per wave).  No lane-divergent
+    // region may precede the broadcast: with `if (lig == 0)` around the atomic
+    // the compiler can give the other lanes a loop of their own that reaches
+    // the broadcast without lane 0's new value, and they never leave it.
     const uint32_t mine = atomicAdd(W.fetch + bin, lig == 0 ? 1u : 0u);
     const uint32_t idx = __shfl(mine, 0, L);
     if (idx >= nlist) break;
@@ -372,20 +378,28 @@ glb_bin_count_kernel(GlbArgs A, GlbWork W) {
   if (threadIdx.x < kGlbBins && hist[threadIdx.x]) atomicAdd(&W.counts[threadIdx.x], hist[threadIdx.x]);
 }
 
+// Ranks within a bin are taken in LDS per 256 tasks and the block reserves
+// each bin's run with one device atomic (a same-address device atomic per
+// task serialises the whole batch on kGlbBins addresses).
 __global__ void __launch_bounds__(256)
 glb_bin_scatter_kernel(int64_t n, GlbWork W) {
-  __shared__ uint32_t off[kGlbBins];
+  __shared__ uint32_t off[kGlbBins], cnt[kGlbBins], run[kGlbBins];
   if (threadIdx.x == 0) {
     uint32_t s = 0;
     for (int b = 0; b < kGlbBins; ++b) { off[b] = s; s += W.counts[b]; }
   }
-  __syncthreads();
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int b = W.bin_of[t];
-    if (b == 0xff) continue;
-    const uint32_t p = atomicAdd(&W.cursor[b], 1u);
-    W.list[off[b] + p] = (int32_t)t;
+  for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x; t0 < n; t0 += (int64_t)gridDim.x * blockDim.x) {
+    if (threadIdx.x < kGlbBins) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t t = t0 + threadIdx.x;
+    const int b = t < n ? W.bin_of[t] : 0xff;
+    uint32_t r = 0;
+    if (b != 0xff) r = atomicAdd(&cnt[b], 1u);
+    __syncthreads();
+    if (threadIdx.x < kGlbBins)
+      run[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&W.cursor[threadIdx.x], cnt[threadIdx.x]) : 0u;
+    __syncthreads();
+    if (b != 0xff) W.list[off[b] + run[b] + r] = (int32_t)t;
   }
 }
 
