@@ -25,7 +25,7 @@ constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; 
 // The CIGAR (ksw_global2) kernel bins tasks the same way with its own shapes.
 constexpr int kGlbBins = 8;
 constexpr BinShape kGlbShape[kGlbBins] = {
-    {16, 1}, {16, 2}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
+    {8, 2}, {8, 4}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
 constexpr int kExtSlices = 8;  // fetch-counter slots per bin (CIGAR bins, lane-kernel bins and classes)
 constexpr int glb_bin_limit(int b) { return kGlbShape[b].lanes * kGlbShape[b].cols; }
 // First entry of fetch slice s of a list of n entries: a balanced partition,
@@ -238,8 +238,10 @@ struct MdArgs {
 };
 hipError_t launch_md(const MdArgs& a, int num_cu, hipStream_t st);
 
-// Blocks per CU of the global kernel; the scratch holds one slot per group.
-constexpr int kGlbBlocksPerCU = 7;
+// Blocks per CU of the global kernel by lanes per task (8-lane shapes hold
+// twice the groups per block, so fewer blocks keep the scratch the same
+// size); the scratch holds one slot per group.
+constexpr int glb_blocks_per_cu(int lanes) { return lanes < 16 ? 4 : 7; }
 int64_t glb_groups(int num_cu);
 hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st);
 

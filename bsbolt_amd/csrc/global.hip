@@ -24,12 +24,21 @@ namespace bsx {
 
 constexpr int kNegInf = -0x40000000;  // MINUS_INF, ksw.c:490
 constexpr int kGlbMaxT = 1 << 16;
-constexpr int kZWinDw = 256;  // dwords of a group's LDS window on its direction bytes
+// dwords of a group's LDS window on its direction bytes (16 KB per block)
+template <int L>
+constexpr int zwin_dw() { return L < 16 ? 128 : 256; }
 
-// Group scans and reductions on DPP (L = 16, 32 or 64 lanes; values may be
+// Group scans and reductions on DPP (L = 8, 16, 32 or 64 lanes; values may be
 // negative, so lanes without a source keep the identity instead of 0).
 template <int L>
 __device__ __forceinline__ int g_incl_max(int x, int ident) {
+  if (L == 8) {  // quad steps take the lane itself where there is no source (max is idempotent)
+    x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x90, 0xf, 0xf, false));  // quad_perm [0,0,1,2]
+    x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x40, 0xf, 0xf, false));  // quad_perm [0,0,0,1]
+    const int q = __builtin_amdgcn_update_dpp(ident, x, 0xFF, 0xf, 0xf, false);  // quad_perm [3,3,3,3]
+    // row_shr:4 into banks 1 and 3 only (lanes 4-7 of each 8-lane group)
+    return max(x, __builtin_amdgcn_update_dpp(ident, q, 0x114, 0xf, 0xa, false));
+  }
   x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
   x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
   x = max(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
@@ -43,7 +52,7 @@ __device__ __forceinline__ int g_incl_max(int x, int ident) {
 template <int L>
 __device__ __forceinline__ int g_excl_max(int x, int lig, int ident) {
   const int incl = g_incl_max<L>(x, ident);
-  const int e = L == 16 ? __builtin_amdgcn_update_dpp(ident, incl, 0x111, 0xf, 0xf, false)   // row_shr:1
+  const int e = L <= 16 ? __builtin_amdgcn_update_dpp(ident, incl, 0x111, 0xf, 0xf, false)   // row_shr:1
                         : __builtin_amdgcn_update_dpp(ident, incl, 0x138, 0xf, 0xf, false);  // wave_shr:1
   return lig == 0 ? ident : e;
 }
@@ -51,10 +60,11 @@ __device__ __forceinline__ int g_excl_max(int x, int lig, int ident) {
 // every lane of the group receives the maximum
 template <int L>
 __device__ __forceinline__ int g_max(int x) {
-  if (L == 16) {
+  if (L <= 16) {
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true));   // quad_perm [1,0,3,2]
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true));   // quad_perm [2,3,0,1]
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, true));  // row_half_mirror
+    if (L == 8) return x;
     return max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, true));  // row_mirror
   }
   x = g_incl_max<L>(x, INT32_MIN);
@@ -66,14 +76,14 @@ __device__ __forceinline__ int g_max(int x) {
 // value of the previous lane of the group (lane 0: unused by the callers)
 template <int L>
 __device__ __forceinline__ int g_prev(int x) {
-  return L == 16 ? __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true)   // row_shr:1
+  return L <= 16 ? __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true)   // row_shr:1
                  : __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true);  // wave_shr:1
 }
 
 // value of the next lane of the group (lane L-1: unused by the callers)
 template <int L>
 __device__ __forceinline__ int g_next(int x) {
-  return L == 16 ? __builtin_amdgcn_update_dpp(0, x, 0x101, 0xf, 0xf, true)   // row_shl:1
+  return L <= 16 ? __builtin_amdgcn_update_dpp(0, x, 0x101, 0xf, 0xf, true)   // row_shl:1
                  : __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true);  // wave_shl:1
 }
 
@@ -89,7 +99,8 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
   const int32_t* __restrict__ list = W.list + lbase;
   const uint32_t nlist = W.counts[bin];
   uint8_t* __restrict__ z = W.z + (int64_t)group * W.zslot;
-  __shared__ uint32_t zwin_all[256 / L][kZWinDw];
+  constexpr int ZW = zwin_dw<L>();
+  __shared__ uint32_t zwin_all[256 / L][ZW];
   uint32_t* __restrict__ zwin = zwin_all[threadIdx.x / L];
   const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
   const int e_del = A.e_del, e_ins = A.e_ins;
@@ -288,7 +299,7 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
       int k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
       // LDS window: rows [wlo, whi] of the slot, from byte wbase (a dword
       // boundary) on; rows per window so that they fit with the alignment slack
-      const int wrows = (4 * kZWinDw - 3) / ncol;
+      const int wrows = (4 * ZW - 3) / ncol;
       const uint32_t* __restrict__ zw = (const uint32_t*)z;
       int wlo = 1, whi = 0;
       int64_t wbase = 0;
@@ -412,13 +423,18 @@ __global__ void glb_reset_kernel(GlbWork W) {
 
 int64_t glb_groups(int num_cu) {
   // the largest group count over the bins is with 16 lanes per task
-  return (int64_t)num_cu * kGlbBlocksPerCU * (256 / 16);
+  int64_t most = 0;
+  for (int b = 0; b < kGlbBins; ++b) {
+    const int64_t g = (int64_t)glb_blocks_per_cu(kGlbShape[b].lanes) * (256 / kGlbShape[b].lanes);
+    most = g > most ? g : most;
+  }
+  return (int64_t)num_cu * most;
 }
 
 template <int B>
 static void glb_launch_bin(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st) {
   constexpr int L = kGlbShape[B].lanes, C = kGlbShape[B].cols;
-  hipLaunchKernelGGL((glb_kernel<L, C>), dim3((unsigned)(num_cu * kGlbBlocksPerCU)), dim3(256), 0, st, a, w,
+  hipLaunchKernelGGL((glb_kernel<L, C>), dim3((unsigned)(num_cu * glb_blocks_per_cu(L))), dim3(256), 0, st, a, w,
                      B);
 }
 

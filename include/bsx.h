@@ -135,7 +135,7 @@ int bsx_global_batch_device(bsx_ctx *ctx, const bsx_scoring *sc,
                             const bsx_glb_tasks *tasks, bsx_glb_results *res);
 /* Bytes of direction scratch per in-flight task (default 65536): a CIGAR task
  * needs min(qlen, 2w+1) * tlen bytes; larger ones are rejected (BSX_EINVAL).
- * The context holds (#CU * 64) such slots of device memory. */
+ * The context holds (#CU * 128) such slots of device memory. */
 void bsx_set_global_zslot(bsx_ctx *ctx, int64_t bytes);
 
 /* ---- NM, MD:Z and XB:Z of CIGARs (the tail of bwa_gen_cigar2, bwa.c:250-343) ----
