@@ -157,29 +157,43 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
 
     // Per-row codes, two rows per lane and 2L rows per chunk: byte 0 the
     // target base of row rho, byte 1 the query base entering the top position
-    // after row rho (column rho + L*C - w; 4 outside the query).  The next
-    // chunk is loaded one chunk ahead.
-    auto chunk = [&](int cb) -> uint32_t {
+    // after row rho (column rho + L*C - w; 4 outside the query).  The raw bytes
+    // of the next chunk are loaded one chunk ahead from clamped (in-bounds)
+    // addresses, unconditionally, so nothing waits for them before the next
+    // refill combines them.
+    uint32_t raw[4];
+    auto load_raw = [&](int cb) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int rho = cb + 2 * lig + k;
+        const int jq = rho + LC - w;
+        raw[2 * k] = A.target[toff + (rho < tlen ? rho : tlen - 1)];
+        raw[2 * k + 1] = A.query[qoff + (jq < 0 ? 0 : (jq < qlen ? jq : qlen - 1))];
+      }
+    };
+    auto combine = [&](int cb) -> uint32_t {
       uint32_t v = 0;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const int rho = cb + 2 * lig + k;
         const int jq = rho + LC - w;
-        const uint32_t tb = rho < tlen ? (uint32_t)A.target[toff + rho] : 0u;
-        const uint32_t qb = (jq >= 0 && jq < qlen) ? (uint32_t)A.query[qoff + jq] : 4u;
+        const uint32_t tb = rho < tlen ? raw[2 * k] : 0u;
+        const uint32_t qb = (jq >= 0 && jq < qlen) ? raw[2 * k + 1] : 4u;
         v |= (tb | qb << 8) << (16 * k);
       }
       return v;
     };
-    uint32_t ch = chunk(0), nx = chunk(2 * L);
+    load_raw(0);
+    uint32_t ch = combine(0);
+    load_raw(2 * L);
     int cb = 0;
     for (int i = 0; i < tlen; ++i) {
       int tr = i - cb;
       if (tr >= 2 * L) {
         cb = i;
         tr = 0;
-        ch = nx;
-        nx = chunk(i + 2 * L);
+        ch = combine(i);
+        load_raw(i + 2 * L);
       }
       const uint32_t wd = (uint32_t)__shfl((int)ch, tr >> 1, L) >> (16 * (tr & 1));
       const int t = (int)(wd & 0xffu), qn = (int)((wd >> 8) & 0xffu);
@@ -194,7 +208,7 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
       for (int c = 0; c < C; ++c) {
         const int r = base + c;
         const bool inb = r < nb;
-        const int s = t < 4 ? (int)((int32_t)(plo[c] << (24 - 8 * t)) >> 24) : phi[c];
+        const int s = t < 4 ? __builtin_amdgcn_sbfe((int)plo[c], 8 * (t & 3), 8) : phi[c];  // signed byte t
         const int mm = G[c] + s;
         M[c] = mm;
         const int key = inb ? mm - oe_ins + r * e_ins : INT32_MIN;
