@@ -23,9 +23,9 @@ constexpr int kNarrowTasksPerBlock = 16;  // ... by one narrow-kernel block
 constexpr int bin_limit(int b) { return kExtShape[b].lanes * kExtShape[b].cols; }
 
 // The CIGAR (ksw_global2) kernel bins tasks the same way with its own shapes.
-constexpr int kGlbBins = 8;
+constexpr int kGlbBins = 9;
 constexpr BinShape kGlbShape[kGlbBins] = {
-    {8, 2}, {8, 4}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
+    {8, 2}, {8, 3}, {8, 4}, {16, 4}, {32, 4}, {64, 4}, {64, 8}, {64, 16}, {64, 32}};
 constexpr int kExtSlices = 8;  // fetch-counter slots per bin (CIGAR bins, lane-kernel bins and classes)
 constexpr int glb_bin_limit(int b) { return kGlbShape[b].lanes * kGlbShape[b].cols; }
 // First entry of fetch slice s of a list of n entries: a balanced partition,

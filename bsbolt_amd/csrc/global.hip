@@ -428,6 +428,7 @@ glb_bin_scatter_kernel(int64_t n, GlbWork W) {
   }
 }
 
+static_assert(kGlbBins <= kCountSlots && kGlbBins <= 64, "CIGAR bin counters");
 __global__ void glb_reset_kernel(GlbWork W) {
   const int k = threadIdx.x;
   if (k < kGlbBins) { W.counts[k] = 0; W.cursor[k] = 0; W.fetch[k] = 0; }
@@ -459,6 +460,7 @@ hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hi
   if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
   hipLaunchKernelGGL(glb_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, w);
   hipLaunchKernelGGL(glb_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a.n, w);
+  glb_launch_bin<8>(a, w, num_cu, st);
   glb_launch_bin<7>(a, w, num_cu, st);
   glb_launch_bin<6>(a, w, num_cu, st);
   glb_launch_bin<5>(a, w, num_cu, st);
