@@ -316,19 +316,23 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
       const int wrows = (4 * ZW - 3) / ncol;
       const uint32_t* __restrict__ zw = (const uint32_t*)z;
       int wlo = 1, whi = 0;
-      int64_t wbase = 0;
+      uint32_t wbase = 0;
+      // Offsets in the slot fit 32 bits: n_col <= 2048 and tlen <= 65536 are
+      // checked per task.  Every lane computes the same (i, k, state), so the
+      // step needs no cross-lane operation.
       while (i >= 0 && k >= 0) {
-        const int lo = i > w ? i - w : 0;
-        if (k < lo || k - lo >= ncol) {  // cannot happen under the preconditions; never read outside the slot
+        const int kc = k - (i > w ? i - w : 0);
+        if ((unsigned)kc >= (unsigned)ncol) {  // cannot happen under the preconditions; never read outside the slot
           over = true;
           break;
         }
-        const int64_t zi = (int64_t)i * ncol + (k - lo);
+        const uint32_t zi = (uint32_t)i * (uint32_t)ncol + (uint32_t)kc;
         int zb;
         if (wrows >= 1) {
           if (i < wlo || i > whi) {  // refill: rows [i - wrows + 1, i], every lane loading
             const int r0 = i - wrows + 1 > 0 ? i - wrows + 1 : 0;
-            const int64_t d0 = ((int64_t)r0 * ncol) >> 2, d1 = ((int64_t)(i + 1) * ncol - 1) >> 2;
+            const uint32_t d0 = ((uint32_t)r0 * (uint32_t)ncol) >> 2;
+            const uint32_t d1 = ((uint32_t)(i + 1) * (uint32_t)ncol - 1u) >> 2;
             __builtin_amdgcn_wave_barrier();
             for (int q = lig; q <= (int)(d1 - d0); q += L) zwin[q] = zw[d0 + q];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -336,16 +340,23 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             wlo = r0; whi = i; wbase = d0 << 2;
           }
-          const int64_t q = zi - wbase;  // every lane reads the same byte
-          zb = (int)(zwin[q >> 2] >> (8 * (q & 3))) & 0xff;
+          const uint32_t q = zi - wbase;  // every lane reads the same byte
+          zb = (int)__builtin_amdgcn_ubfe(zwin[q >> 2], 8 * (q & 3), 8);
         } else {
-          // lane 0's byte, broadcast: every lane takes the same step by construction
-          zb = __shfl((int)z[zi], 0, L);
+          zb = z[zi];  // every lane loads the same byte
         }
         state = zb >> (state << 1) & 3;
-        if (state == 0) { push(0, 1); --i; --k; }
-        else if (state == 1) { push(2, 1); --i; }
-        else { push(1, 1); --k; }
+        // 0: M, i-1 and k-1; 1: D, i-1; otherwise I, k-1 (ksw.c:596-598)
+        const int op = state == 0 ? 0 : (state == 1 ? 2 : 1);
+        i -= state <= 1 ? 1 : 0;
+        k -= state != 1 ? 1 : 0;
+        if (op != run_op) {  // the run of run_op ends (there is none before the first step)
+          emit();
+          run_op = op;
+          run_len = 1;
+        } else {
+          ++run_len;
+        }
       }
       if (i >= 0) push(2, i + 1);
       if (k >= 0) push(1, k + 1);
