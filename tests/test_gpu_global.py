@@ -113,3 +113,38 @@ def test_global_long_queries_narrow_and_wide_bands(engine):
     assert np.array_equal(ncig, on)
     for i in range(len(tk)):
         assert np.array_equal(cig[i, :ncig[i]], oc[i, :on[i]]), f"task {i}"
+
+
+def test_global_wide_bands_walk_paths():
+    """Bands wider than the walk's LDS window (n_col > 1021: every lane loads
+    the direction byte) and bands of one window row per refill (n_col 513-1024),
+    on an engine of their own: the direction scratch grows to #CU * 128 slots of
+    ~1.1 MB for this test only."""
+    from bsbolt_amd.bsx import Engine
+    from bsbolt_amd.tasks import GlbTasks
+    rng = np.random.default_rng(46)
+    pairs, ws = [], []
+    for ql, w in [(1030, 520)] * 4 + [(800, 300)] * 4:
+        q = rng.integers(0, 5, ql).astype(np.uint8)
+        t = list(np.where(rng.random(ql) < 0.05, rng.integers(0, 4, ql), q % 4))
+        for _ in range(3):
+            p = int(rng.integers(0, len(t) + 1)); L = int(rng.integers(1, 4))
+            if rng.random() < 0.5:
+                t[p:p] = list(rng.integers(0, 4, L))
+            else:
+                del t[p:p + L]
+        pairs.append((q, np.array(t, np.uint8))); ws.append(w)
+    tk = GlbTasks.from_pairs(pairs, ws)
+    slot = int((tk.qlen + tk.tlen).max())
+    need = int((np.minimum(tk.qlen.astype(np.int64), 2 * tk.w.astype(np.int64) + 1) * tk.tlen).max())
+    e = Engine(0)
+    try:
+        e.set_global_zslot(need + 64)
+        score, ncig, cig = e.global_(tk, slot=slot)
+    finally:
+        e.close()
+    os_, on, oc, _ = O.global2_batch(tk, slot)
+    assert np.array_equal(score, os_)
+    assert np.array_equal(ncig, on)
+    for i in range(len(tk)):
+        assert np.array_equal(cig[i, :ncig[i]], oc[i, :on[i]]), f"task {i}"
