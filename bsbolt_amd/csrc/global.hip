@@ -448,7 +448,7 @@ __global__ void glb_reset_kernel(GlbWork W) {
 
 
 int64_t glb_groups(int num_cu) {
-  // the largest group count over the bins is with 16 lanes per task
+  // one direction-byte slot per group of the largest grid over the shapes
   int64_t most = 0;
   for (int b = 0; b < kGlbBins; ++b) {
     const int64_t g = (int64_t)glb_blocks_per_cu(kGlbShape[b].lanes) * (256 / kGlbShape[b].lanes);
