@@ -534,25 +534,74 @@ double orc_extend2_batch(void *fn, const int8_t *mat, int o_del, int e_del, int 
 }
 
 /* Global (CIGAR) batch: per-task band w; CIGARs go to fixed slots of
- * `slot` uint32 per task (n_cigar[t] = -1 if the CIGAR did not fit). */
-double orc_global2_batch(const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins,
-                         int64_t n, const uint8_t *query, const uint64_t *q_off, const int32_t *qlen,
-                         const uint8_t *target, const uint64_t *t_off, const int32_t *tlen,
-                         const int32_t *w, int32_t *score, int32_t *n_cigar, uint32_t *cigar, int64_t slot)
+ * `slot` uint32 per task (n_cigar[t] = -1 if the CIGAR did not fit).  Threads
+ * pull dynamic chunks of 4096 tasks as the extension driver does; `fn` NULL
+ * runs orc_global2, otherwise a routine with ksw_global2's signature
+ * (ksw.h:84), e.g. the compiled reference.  Returns the wall-clock seconds. */
+typedef int (*orc_glb_fn)(int, const uint8_t *, int, const uint8_t *, int, const int8_t *,
+                          int, int, int, int, int, int *, uint32_t **);
+
+typedef struct {
+	const int8_t *mat; int o_del, e_del, o_ins, e_ins;
+	int64_t n, slot;
+	const uint8_t *query; const uint64_t *q_off; const int32_t *qlen;
+	const uint8_t *target; const uint64_t *t_off; const int32_t *tlen; const int32_t *w;
+	int32_t *score, *n_cigar; uint32_t *cigar;
+	orc_glb_fn fn;
+	int64_t next;
+	pthread_mutex_t mu;
+} glb_job;
+
+static void *glb_worker(void *arg)
 {
-	double t0 = now_s();
-	for (int64_t t = 0; t < n; ++t) {
-		int nc = 0; uint32_t *cg = 0;
-		int s = orc_global2(qlen[t], query + q_off[t], tlen[t], target + t_off[t], 5, mat,
-		                    o_del, e_del, o_ins, e_ins, w[t], cigar ? &nc : 0, cigar ? &cg : 0);
-		score[t] = s;
-		if (cigar) {
-			if (nc <= slot) { memcpy(cigar + t * slot, cg, (size_t)nc * 4); n_cigar[t] = nc; }
-			else n_cigar[t] = -1;
-			free(cg);
+	glb_job *J = (glb_job *)arg;
+	for (;;) {
+		pthread_mutex_lock(&J->mu);
+		int64_t b = J->next; J->next += 4096;
+		pthread_mutex_unlock(&J->mu);
+		if (b >= J->n) break;
+		int64_t e = b + 4096 < J->n ? b + 4096 : J->n;
+		for (int64_t t = b; t < e; ++t) {
+			int nc = 0; uint32_t *cg = 0;
+			int *pnc = J->cigar ? &nc : 0; uint32_t **pcg = J->cigar ? &cg : 0;
+			int s = J->fn ? J->fn(J->qlen[t], J->query + J->q_off[t], J->tlen[t], J->target + J->t_off[t], 5,
+			                      J->mat, J->o_del, J->e_del, J->o_ins, J->e_ins, J->w[t], pnc, pcg)
+			              : orc_global2(J->qlen[t], J->query + J->q_off[t], J->tlen[t], J->target + J->t_off[t], 5,
+			                            J->mat, J->o_del, J->e_del, J->o_ins, J->e_ins, J->w[t], pnc, pcg);
+			J->score[t] = s;
+			if (J->cigar) {
+				if (nc <= J->slot) { memcpy(J->cigar + t * J->slot, cg, (size_t)nc * 4); J->n_cigar[t] = nc; }
+				else J->n_cigar[t] = -1;
+				free(cg);
+			}
 		}
 	}
-	return now_s() - t0;
+	return 0;
+}
+
+double orc_global2_batch(void *fn, const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins,
+                         int64_t n, const uint8_t *query, const uint64_t *q_off, const int32_t *qlen,
+                         const uint8_t *target, const uint64_t *t_off, const int32_t *tlen,
+                         const int32_t *w, int32_t *score, int32_t *n_cigar, uint32_t *cigar, int64_t slot,
+                         int nthreads)
+{
+	glb_job J;
+	memset(&J, 0, sizeof J);
+	J.mat = mat; J.o_del = o_del; J.e_del = e_del; J.o_ins = o_ins; J.e_ins = e_ins;
+	J.n = n; J.slot = slot; J.query = query; J.q_off = q_off; J.qlen = qlen;
+	J.target = target; J.t_off = t_off; J.tlen = tlen; J.w = w;
+	J.score = score; J.n_cigar = n_cigar; J.cigar = cigar; J.fn = (orc_glb_fn)fn; J.next = 0;
+	pthread_mutex_init(&J.mu, 0);
+	if (nthreads < 1) nthreads = 1;
+	if (nthreads > 256) nthreads = 256;
+	pthread_t th[256];
+	double t0 = now_s();
+	for (int k = 1; k < nthreads; ++k) pthread_create(&th[k], 0, glb_worker, &J);
+	glb_worker(&J);
+	for (int k = 1; k < nthreads; ++k) pthread_join(th[k], 0);
+	double dt = now_s() - t0;
+	pthread_mutex_destroy(&J.mu);
+	return dt;
 }
 
 /* orc_gen_md over a batch on nthreads host threads (dynamic chunks of 4096

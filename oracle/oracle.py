@@ -40,8 +40,8 @@ def lib():
         L.orc_extend2_batch.argtypes = [_P, _P, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P,
                                         _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I]
         L.orc_global2_batch.restype = C.c_double
-        L.orc_global2_batch.argtypes = [_P, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P,
-                                        _P, _P, _P, _I64]
+        L.orc_global2_batch.argtypes = [_P, _P, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                        _P, _P, _P, _I64, _I]
         L.orc_cal_max_gap.restype = _I
         L.orc_cal_max_gap.argtypes = [_I] * 7
         L.orc_global_band.restype = _I
@@ -239,18 +239,23 @@ def extend2_batch(b, nthreads=1, use_ref=False, scoring=None, want_cells=True):
     return outs, dt
 
 
-def global2_batch(b, slot, scoring=None):
+def global2_batch(b, slot, scoring=None, nthreads=1, use_ref=False):
+    """ksw_global2 with CIGARs over a task batch (attributes query, q_off, qlen,
+    target, t_off, tlen, w) on `nthreads` host threads: this restatement, or the
+    compiled reference with use_ref.  Returns (score, n_cigar, cigar[n, slot],
+    seconds)."""
     sc = scoring or {}
     mat = np.ascontiguousarray(sc.get("mat", fill_scmat()), np.int8)
     n = len(b.qlen)
     score = np.zeros(n, np.int32)
     ncig = np.zeros(n, np.int32)
     cig = np.zeros(n * slot, np.uint32)
+    fn = C.cast(ref().ksw_global2, _P) if use_ref else None
     arrs = [np.ascontiguousarray(getattr(b, k)) for k in
             ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")]
-    dt = lib().orc_global2_batch(_ptr(mat), sc.get("o_del", 6), sc.get("e_del", 1), sc.get("o_ins", 6),
+    dt = lib().orc_global2_batch(fn, _ptr(mat), sc.get("o_del", 6), sc.get("e_del", 1), sc.get("o_ins", 6),
                                  sc.get("e_ins", 1), n, *[_ptr(a) for a in arrs], _ptr(score), _ptr(ncig),
-                                 _ptr(cig), slot)
+                                 _ptr(cig), slot, nthreads)
     return score, ncig, cig.reshape(n, slot), dt
 
 

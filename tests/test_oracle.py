@@ -164,3 +164,33 @@ def test_oracle_vs_compiled_reference_random_params():
         args = (q, t, int(rng.integers(1, 250)), int(rng.integers(0, 210)), int(rng.integers(0, 40)),
                 int(rng.choice([0, 15, 100])))
         assert O.extend2(*args, **kw)[:6] == O.extend2(*args, **kw, use_ref=True)[:6]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference (oracle/_ref) not present")
+def test_global2_batch_threads_and_compiled_reference():
+    """The threaded batch driver gives the same scores and CIGARs on 1 and 8
+    threads, through the restatement and through the compiled reference's
+    ksw_global2, over 9000 tasks (more than two 4096-task chunks) with indels
+    and bands from 0 to 40."""
+    from bsbolt_amd.tasks import GlbTasks
+    rng = np.random.default_rng(5)
+    pairs, ws = [], []
+    for _ in range(9000):
+        ql = int(rng.integers(1, 120))
+        q = rng.integers(0, 5, ql).astype(np.uint8)
+        t = list(np.where(rng.random(ql) < 0.05, rng.integers(0, 4, ql), q % 4))
+        for _ in range(int(rng.integers(0, 3))):
+            p = int(rng.integers(0, len(t) + 1)); L = int(rng.integers(1, 4))
+            if rng.random() < 0.5:
+                t[p:p] = list(rng.integers(0, 4, L))
+            else:
+                del t[p:p + L]
+        t = np.array(t or [1], np.uint8)
+        pairs.append((q, t)); ws.append(abs(len(t) - ql) + int(rng.integers(0, 40)))
+    tk = GlbTasks.from_pairs(pairs, ws)
+    slot = int((tk.qlen + tk.tlen).max())
+    base = O.global2_batch(tk, slot)
+    for kw in (dict(nthreads=8), dict(use_ref=True), dict(nthreads=8, use_ref=True)):
+        got = O.global2_batch(tk, slot, **kw)
+        for k in range(3):
+            assert np.array_equal(base[k], got[k]), kw
