@@ -105,6 +105,11 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
   const int oe_del = A.o_del + A.e_del, oe_ins = A.o_ins + A.e_ins;
   const int e_del = A.e_del, e_ins = A.e_ins;
 
+  // profile words by query code (code 4 for N and positions past the query),
+  // for the position entering the band on each row
+  __shared__ uint2 s_prof[5];
+  if (threadIdx.x < 5) s_prof[threadIdx.x] = make_uint2(A.prof_lo[threadIdx.x], (uint32_t)A.prof_hi[threadIdx.x]);
+  __syncthreads();
   // An empty bin returns before its first fetch (uniform over the grid): each
   // wave's fetch is a device atomic on one address, ~10 ns apiece serialised.
   if (nlist == 0) return;
@@ -254,16 +259,11 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
           plo[c] = plo[c + 1];
           phi[c] = phi[c + 1];
         }
-        uint32_t lo = A.prof_lo[4];
-        int hi = A.prof_hi[4];
-        if (qn == 0) { lo = A.prof_lo[0]; hi = A.prof_hi[0]; }
-        if (qn == 1) { lo = A.prof_lo[1]; hi = A.prof_hi[1]; }
-        if (qn == 2) { lo = A.prof_lo[2]; hi = A.prof_hi[2]; }
-        if (qn == 3) { lo = A.prof_lo[3]; hi = A.prof_hi[3]; }
+        const uint2 pq = s_prof[qn < 4 ? qn : 4];
         const bool top = lig == L - 1;
         E[C - 1] = top ? kNegInf : e0;
-        plo[C - 1] = top ? lo : p0;
-        phi[C - 1] = top ? hi : q0;
+        plo[C - 1] = top ? pq.x : p0;
+        phi[C - 1] = top ? (int)pq.y : q0;
       } else {  // beg stays 0: H(i,j-1) moves to position j, eh[0].h = h1p
         const int hprev = g_prev<L>(h[C - 1]);
 #pragma unroll
