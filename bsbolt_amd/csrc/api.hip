@@ -725,9 +725,10 @@ int bsx_global_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tas
   wk.err = (int32_t*)(wk.cursor + kCountSlots);
   wk.fetch = (uint32_t*)(wk.err + 4);
   wk.zslot = c->zslot;
+  wk.ss = c->ss_on ? &c->ss : nullptr;
   wk.z = nullptr;
   if (r->cigar) {
-    HIPCHK(c, c->ws->zbuf.ensure((size_t)glb_groups(c->num_cu) * (size_t)c->zslot), "hipMalloc(z)");
+    HIPCHK(c, c->ws->zbuf.ensure((size_t)glb_groups(c->num_cu, c->ss_on) * (size_t)c->zslot), "hipMalloc(z)");
     wk.z = (uint8_t*)c->ws->zbuf.p;
   }
   HIPCHK(c, launch_global_impl(a, wk, c->num_cu, c->stream), "global launch");
@@ -879,7 +880,7 @@ int bsx_graph_capture_global(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_ta
     HIPCHK(c, c->ws->bin_of.ensure(n + 1), "hipMalloc(bin_of)");
     HIPCHK(c, c->ws->list.ensure(4 * n + 4), "hipMalloc(list)");
     HIPCHK(c, c->ws->small.ensure(kSmallBytes), "hipMalloc(work)");
-    if (r->cigar) HIPCHK(c, c->ws->zbuf.ensure((size_t)glb_groups(c->num_cu) * (size_t)c->zslot), "hipMalloc(z)");
+    if (r->cigar) HIPCHK(c, c->ws->zbuf.ensure((size_t)glb_groups(c->num_cu, c->ss_on) * (size_t)c->zslot), "hipMalloc(z)");
     return BSX_OK;
   };
   return capture(c, 2, sized, [&]() { return bsx_global_batch_device(c, sc, t, r); }, out);

@@ -215,6 +215,7 @@ struct GlbWork {
   int32_t* err;  // [0] rejected, [1] first rejected, [2] CIGAR overflows
   uint8_t* z;    // per-group direction-byte slots
   int64_t zslot; // bytes per slot
+  const StreamSet* ss;  // side streams for the bins with few tasks per group (nullptr: one stream)
 };
 
 // NM / MD:Z / XB:Z of CIGARs (bwa_gen_cigar2's tail, bwa.c:250-343); fields as
@@ -242,7 +243,7 @@ hipError_t launch_md(const MdArgs& a, int num_cu, hipStream_t st);
 // twice the groups per block, so fewer blocks keep the scratch the same
 // size); the scratch holds one slot per group.
 constexpr int glb_blocks_per_cu(int lanes) { return lanes < 16 ? 4 : 7; }
-int64_t glb_groups(int num_cu);
+int64_t glb_groups(int num_cu, bool side);  // scratch slots, with or without the side-stream region
 hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st);
 
 }  // namespace bsx

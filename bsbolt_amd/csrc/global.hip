@@ -89,7 +89,7 @@ __device__ __forceinline__ int g_next(int x) {
 
 template <int L, int C>
 __global__ void __launch_bounds__(256)
-glb_kernel(GlbArgs A, GlbWork W, int bin) {
+glb_kernel(GlbArgs A, GlbWork W, int bin, int64_t zbase) {
   const int lane = threadIdx.x & 63;
   const int lig = lane & (L - 1);
   const int base = lig * C;
@@ -98,7 +98,7 @@ glb_kernel(GlbArgs A, GlbWork W, int bin) {
   for (int b = 0; b < bin; ++b) lbase += W.counts[b];
   const int32_t* __restrict__ list = W.list + lbase;
   const uint32_t nlist = W.counts[bin];
-  uint8_t* __restrict__ z = W.z + (int64_t)group * W.zslot;
+  uint8_t* __restrict__ z = W.z + (zbase + group) * W.zslot;  // zbase: this launch's slot region
   constexpr int ZW = zwin_dw<L>();
   __shared__ uint32_t zwin_all[256 / L][ZW];
   uint32_t* __restrict__ zwin = zwin_all[threadIdx.x / L];
@@ -447,21 +447,39 @@ __global__ void glb_reset_kernel(GlbWork W) {
 }
 
 
-int64_t glb_groups(int num_cu) {
-  // one direction-byte slot per group of the largest grid over the shapes
+// The 8x4 and 16x4 bins (bands 25-64 wide) hold about 1.5 tasks per group in
+// a CIGAR pass, so their kernels are mostly tail.  With side streams they run
+// beside the 8x2/8x3 bins on a side stream, with fewer blocks and a slot
+// region of their own after the main one.
+constexpr int kGlbSideBlocksPerCU = 2;
+constexpr bool glb_on_side(int b) { return b == 2 || b == 3; }
+
+static int64_t glb_groups_per_cu(bool side) {
   int64_t most = 0;
   for (int b = 0; b < kGlbBins; ++b) {
-    const int64_t g = (int64_t)glb_blocks_per_cu(kGlbShape[b].lanes) * (256 / kGlbShape[b].lanes);
+    if (glb_on_side(b) != side) continue;
+    const int bpc = side ? kGlbSideBlocksPerCU : glb_blocks_per_cu(kGlbShape[b].lanes);
+    const int64_t g = (int64_t)bpc * (256 / kGlbShape[b].lanes);
     most = g > most ? g : most;
   }
-  return (int64_t)num_cu * most;
+  return most;
+}
+
+int64_t glb_groups(int num_cu, bool side) {
+  // one direction-byte slot per group: the main region covers the largest grid
+  // of the call's stream (and every bin when side streams are off), the side
+  // region the side-stream grids
+  return (int64_t)num_cu * (glb_groups_per_cu(false) + (side ? glb_groups_per_cu(true) : 0));
 }
 
 template <int B>
-static void glb_launch_bin(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st) {
+static void glb_launch_bin(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t main) {
   constexpr int L = kGlbShape[B].lanes, C = kGlbShape[B].cols;
-  hipLaunchKernelGGL((glb_kernel<L, C>), dim3((unsigned)(num_cu * glb_blocks_per_cu(L))), dim3(256), 0, st, a, w,
-                     B);
+  const bool side = glb_on_side(B) && w.ss != nullptr;
+  const hipStream_t st = side ? pick_stream(w.ss, main, 1) : main;
+  const int bpc = side ? kGlbSideBlocksPerCU : glb_blocks_per_cu(L);
+  const int64_t zbase = side ? (int64_t)num_cu * glb_groups_per_cu(false) : 0;
+  hipLaunchKernelGGL((glb_kernel<L, C>), dim3((unsigned)(num_cu * bpc)), dim3(256), 0, st, a, w, B, zbase);
 }
 
 hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hipStream_t st) {
@@ -471,15 +489,17 @@ hipError_t launch_global_impl(const GlbArgs& a, const GlbWork& w, int num_cu, hi
   if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
   hipLaunchKernelGGL(glb_bin_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, w);
   hipLaunchKernelGGL(glb_bin_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a.n, w);
+  ss_fork(w.ss, st);  // side streams start after the scatter
+  glb_launch_bin<3>(a, w, num_cu, st);
+  glb_launch_bin<2>(a, w, num_cu, st);
   glb_launch_bin<8>(a, w, num_cu, st);
   glb_launch_bin<7>(a, w, num_cu, st);
   glb_launch_bin<6>(a, w, num_cu, st);
   glb_launch_bin<5>(a, w, num_cu, st);
   glb_launch_bin<4>(a, w, num_cu, st);
-  glb_launch_bin<3>(a, w, num_cu, st);
-  glb_launch_bin<2>(a, w, num_cu, st);
   glb_launch_bin<1>(a, w, num_cu, st);
   glb_launch_bin<0>(a, w, num_cu, st);
+  ss_join(w.ss, st);  // the call's stream waits for the side streams
   return hipGetLastError();
 }
 

@@ -135,7 +135,8 @@ int bsx_global_batch_device(bsx_ctx *ctx, const bsx_scoring *sc,
                             const bsx_glb_tasks *tasks, bsx_glb_results *res);
 /* Bytes of direction scratch per in-flight task (default 65536): a CIGAR task
  * needs min(qlen, 2w+1) * tlen bytes; larger ones are rejected (BSX_EINVAL).
- * The context holds (#CU * 128) such slots of device memory. */
+ * The context holds #CU * 192 such slots of device memory (#CU * 128 with
+ * side streams off, bsx_set_side_streams). */
 void bsx_set_global_zslot(bsx_ctx *ctx, int64_t bytes);
 
 /* ---- NM, MD:Z and XB:Z of CIGARs (the tail of bwa_gen_cigar2, bwa.c:250-343) ----
@@ -277,10 +278,11 @@ int bsx_kernel_times(bsx_ctx *ctx, float *ms, int n_slots, int *calls);
 int bsx_ext_bin_limits(int32_t *limits, int n);
 /* bsx_set_side_streams(ctx, 0|1): the extension calls spread their independent
  * launches (query-length bins, frame classes) over the context's stream and
- * three side streams of its own (1, the default), or run them one after the
- * other on the context's stream (0: each kernel then has the GPU to itself,
- * which is how per-kernel times are taken for the roofline).  No counterpart
- * in the reference. */
+ * three side streams of its own, and the CIGAR calls run their band-width bins
+ * 25-64 on a side stream beside the others (1, the default); or every launch
+ * runs one after the other on the context's stream (0: each kernel then has
+ * the GPU to itself, which is how per-kernel times are taken for the
+ * roofline).  No counterpart in the reference. */
 int bsx_set_side_streams(bsx_ctx *ctx, int enable);
 
 /* Scalar drop-in with the exact signature of ksw_extend2 (ksw.h:108), run as

@@ -139,6 +139,7 @@ def test_global_wide_bands_walk_paths():
     need = int((np.minimum(tk.qlen.astype(np.int64), 2 * tk.w.astype(np.int64) + 1) * tk.tlen).max())
     e = Engine(0)
     try:
+        e.L.bsx_set_side_streams(e.ctx, 0)  # no side-stream slot region: #CU * 128 slots
         e.set_global_zslot(need + 64)
         score, ncig, cig = e.global_(tk, slot=slot)
     finally:
