@@ -454,7 +454,7 @@ __global__ void glb_reset_kernel(GlbWork W) {
 constexpr int kGlbSideBlocksPerCU = 2;
 constexpr bool glb_on_side(int b) { return b == 2 || b == 3; }
 
-static int64_t glb_groups_per_cu(bool side) {
+constexpr int64_t glb_groups_per_cu(bool side) {
   int64_t most = 0;
   for (int b = 0; b < kGlbBins; ++b) {
     if (glb_on_side(b) != side) continue;
@@ -464,6 +464,18 @@ static int64_t glb_groups_per_cu(bool side) {
   }
   return most;
 }
+
+// with side streams off every bin runs on the call's stream at its own block
+// count, inside the main region
+constexpr int64_t glb_groups_per_cu_one_stream() {
+  int64_t most = 0;
+  for (int b = 0; b < kGlbBins; ++b) {
+    const int64_t g = (int64_t)glb_blocks_per_cu(kGlbShape[b].lanes) * (256 / kGlbShape[b].lanes);
+    most = g > most ? g : most;
+  }
+  return most;
+}
+static_assert(glb_groups_per_cu(false) >= glb_groups_per_cu_one_stream(), "main slot region covers every bin");
 
 int64_t glb_groups(int num_cu, bool side) {
   // one direction-byte slot per group: the main region covers the largest grid
