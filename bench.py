@@ -36,6 +36,32 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host cores the CPU baseline may use and how they were counted: the
+    CPUs this process may run on (sched_getaffinity), capped by OMP_NUM_THREADS
+    where the job environment sets its CPU share there."""
+    aff = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = min(aff, int(share)) if share and share.isdigit() and int(share) > 0 else aff
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity": aff,
+            "omp_num_threads": share, "model": model}
+
+
+def cpu_fields(args):
+    """Host description carried by every cpu_baseline object."""
+    h = host_cpu()
+    return {"cores": args.cpu_threads, "nproc": h["nproc"], "affinity_cpus": h["affinity"],
+            "job_cpu_share": h["omp_num_threads"], "cpu_model": h["model"]}
+
+
 def profiled_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (calibrated against a copy of known size), or None."""
@@ -172,7 +198,7 @@ def cigar_leg(args, eng, gt):
         mism = int((got_s != sc_).sum() + (got_n != nc_).sum())
         mism += sum(not np.array_equal(got_c[i, :nc_[i]], cg_[i, :nc_[i]]) for i in range(m) if nc_[i] > 0)
         kind = "reference" if use_ref else "port"
-        entry["cpu_baseline"] = {"value": n / dt, "unit": "alignments/s", "cores": args.cpu_threads, "kind": kind,
+        entry["cpu_baseline"] = {"value": n / dt, "unit": "alignments/s", **cpu_fields(args), "kind": kind,
                                  "sample": f"the whole batch ({n} tasks) on {args.cpu_threads} threads, dynamic "
                                            f"4096-task chunks; scores and CIGAR counts compared on every task, "
                                            f"CIGARs on the {m} distinct ones",
@@ -239,7 +265,7 @@ def md_leg(args, eng):
                    + (ref["meth"] != outs["meth"].numpy()[:m]).any(axis=1).sum())
         mism += sum(ref["md"][i] != bytes(gmd[i, :ml[i]]).decode() or ref["xb"][i] != bytes(gxb[i, :xl[i]]).decode()
                     for i in range(m))
-        cpu = {"value": m / dt, "unit": "alignments/s", "cores": args.cpu_threads, "kind": "port",
+        cpu = {"value": m / dt, "unit": "alignments/s", **cpu_fields(args), "kind": "port",
                "sample": f"the {m} distinct regions, oracle restatement of bwa.c:250-343", "seconds": dt,
                "mismatches_vs_gpu": int(mism)}
         log(f"NM/MD/XB CPU restatement: {m / dt / 1e6:.2f} M regions/s on {args.cpu_threads} threads, "
@@ -261,7 +287,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="cfg1")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload's reads")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=host_cpu()["threads"],
+                    help="CPU-baseline threads (default: every CPU this job may use)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-band leg")
     ap.add_argument("--no-md", action="store_true", help="skip the NM/MD/XB leg")
@@ -429,7 +456,7 @@ def main():
         cpu_out, dt = O.extend2_batch(batch, nthreads=args.cpu_threads, use_ref=(kind == "reference"),
                                       want_cells=False)
         mism = int(sum(int((cpu_out[k] != got[k]).sum()) for k in RESULT_FIELDS))
-        line["cpu_baseline"] = {"value": n / dt, "unit": "extensions/s", "cores": args.cpu_threads,
+        line["cpu_baseline"] = {"value": n / dt, "unit": "extensions/s", **cpu_fields(args),
                                 "kind": kind, "gcups": total_cells / dt / 1e9, "seconds": dt,
                                 "sample": f"the full batch ({n} tasks), dynamic 4096-task chunks per thread",
                                 "mismatches_vs_gpu": mism}

@@ -35,7 +35,7 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
-           "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device")
+           "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device", "bsx_dropin_contexts")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -174,6 +174,8 @@ def lib():
             getattr(L, f).restype = C.c_int
         L.bsx_ksw_extend2.restype = C.c_int
         L.bsx_ksw_extend2.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P] + [C.c_int] * 8 + [P] * 5
+        L.bsx_dropin_contexts.restype = C.c_int64
+        L.bsx_dropin_contexts.argtypes = []
         _lib = L
     return _lib
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -907,12 +908,48 @@ void bsx_graph_destroy(bsx_graph* g) {
   delete g;
 }
 
+// Contexts of the scalar drop-in.  A call borrows one from a process-wide
+// free list and returns it, so the number of contexts is the peak number of
+// concurrent callers, not the number of threads that ever called: the
+// reference's kt_for (kthread.c:49-58) starts fresh worker threads for every
+// batch of reads, and a context per thread would grow without bound.  The
+// contexts live until the process ends.
+static std::mutex g_dropin_mu;
+static std::vector<bsx_ctx*> g_dropin_free;
+static int64_t g_dropin_made = 0;
+
+static bsx_ctx* dropin_acquire() {
+  {
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    if (!g_dropin_free.empty()) {
+      bsx_ctx* c = g_dropin_free.back();
+      g_dropin_free.pop_back();
+      return c;
+    }
+  }
+  bsx_ctx* c = bsx_create(0);
+  if (c) {
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    ++g_dropin_made;
+  }
+  return c;
+}
+
+static void dropin_release(bsx_ctx* c) {
+  std::lock_guard<std::mutex> lk(g_dropin_mu);
+  g_dropin_free.push_back(c);
+}
+
+int64_t bsx_dropin_contexts(void) {
+  std::lock_guard<std::mutex> lk(g_dropin_mu);
+  return g_dropin_made;
+}
+
 int bsx_ksw_extend2(int qlen, const uint8_t* query, int tlen, const uint8_t* target, int m,
                     const int8_t* mat, int o_del, int e_del, int o_ins, int e_ins, int w, int end_bonus,
                     int zdrop, int h0, int* qle, int* tle, int* gtle, int* gscore, int* max_off) {
-  static thread_local bsx_ctx* ctx = nullptr;
   if (m != 5 || !query || !mat || (tlen > 0 && !target)) return INT32_MIN;
-  if (!ctx) ctx = bsx_create(0);
+  bsx_ctx* ctx = dropin_acquire();
   if (!ctx) return INT32_MIN;
   bsx_scoring sc;
   memcpy(sc.mat, mat, 25);
@@ -922,7 +959,9 @@ int bsx_ksw_extend2(int qlen, const uint8_t* query, int tlen, const uint8_t* tar
   bsx_ext_tasks t{1, query, &zero, &qlen, tlen > 0 ? target : &dummy, &zero, &tlen, &h0, &w, &end_bonus};
   int32_t o[6];
   bsx_ext_results r{&o[0], &o[1], &o[2], &o[3], &o[4], &o[5], nullptr};
-  if (bsx_extend_batch(ctx, &sc, &t, &r) != BSX_OK) return INT32_MIN;
+  const int rc = bsx_extend_batch(ctx, &sc, &t, &r);
+  dropin_release(ctx);
+  if (rc != BSX_OK) return INT32_MIN;
   if (qle) *qle = o[1];
   if (tle) *tle = o[2];
   if (gtle) *gtle = o[3];

@@ -238,7 +238,7 @@ ext_lane_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32_t
       }
       if (plo == phi && nlo < nhi) { plo = nlo; phi = nhi; nlo = nhi = 0; }
       if (!apend && !exhausted && nlo == nhi && phi - plo < CH / 2) {
-        if (lane == 0) aret = atomicAdd(&fslot[cs], CH);  // every lane is active here
+        aret = atomicAdd(&fslot[cs], lane == 0 ? (uint32_t)CH : 0u);  // branch-free; lane 0's value is read
         apend = true;
         acs = cs;
       }
@@ -578,7 +578,7 @@ ext_lane_frame_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const ui
       }
       if (plo == phi && nlo < nhi) { plo = nlo; phi = nhi; nlo = nhi = 0; }
       if (!apend && !exhausted && nlo == nhi && phi - plo < CH / 2) {
-        if (lane == 0) aret = atomicAdd(&fslot[cs], CH);
+        aret = atomicAdd(&fslot[cs], lane == 0 ? (uint32_t)CH : 0u);
         apend = true;
         acs = cs;
       }

@@ -113,8 +113,11 @@ ext_narrow_kernel(ExtKArgs A, const ExtDesc* __restrict__ desc_all, const uint32
   const int oe_del = A.o_del + e_del, oe_ins = A.o_ins + e_ins;
 
   auto next_pos = [&]() -> uint32_t {
-    uint32_t mine = 0;
-    if (lig == 0) mine = atomicAdd(&lds_next, 1u);
+    // Branch-free fetch: every lane issues the atomic and only lane 0 of the
+    // group adds 1.  No lane-divergent region may precede the broadcast: with
+    // `if (lig == 0)` around the atomic the compiler can give the other lanes a
+    // loop of their own that reaches the broadcast without lane 0's new value.
+    const uint32_t mine = atomicAdd(&lds_next, lig == 0 ? 1u : 0u);
     const uint32_t p = __shfl(mine, 0, L);
     return p < blk1 ? p : blk1;
   };

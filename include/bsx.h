@@ -286,13 +286,23 @@ int bsx_ext_bin_limits(int32_t *limits, int n);
 int bsx_set_side_streams(bsx_ctx *ctx, int enable);
 
 /* Scalar drop-in with the exact signature of ksw_extend2 (ksw.h:108), run as
- * a one-task batch on the calling thread's default context (device 0).  m must
- * be 5.  On a precondition violation or device error it returns INT32_MIN and
- * leaves the outputs untouched (the reference asserts instead, ksw.c:385). */
+ * a one-task batch on device 0.  m must be 5.  It returns INT32_MIN and leaves
+ * *qle, *tle, *gtle, *gscore and *max_off untouched in these cases, which the
+ * caller must check for (mem_chain2aln, bwamem.c:735,763, reads all five):
+ *  - a precondition of bsx_ext_tasks fails: h0 <= 0 (the reference asserts,
+ *    ksw.c:385), qlen < 1 (undefined in the reference, ksw.c:388,395),
+ *    qlen > BSX_MAX_QLEN, h0 + qlen*max(mat) >= 2^20, w < 0, tlen < 0;
+ *  - m != 5, query or mat NULL, or target NULL with tlen > 0;
+ *  - no device, or a device error (bsx_create or the batch call fails).
+ * Calls from many threads are safe: each call borrows a context from a
+ * process-wide pool and returns it, so the pool holds as many contexts as the
+ * peak number of concurrent callers (contexts live until the process ends). */
 int bsx_ksw_extend2(int qlen, const uint8_t *query, int tlen, const uint8_t *target,
                     int m, const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins,
                     int w, int end_bonus, int zdrop, int h0,
                     int *qle, int *tle, int *gtle, int *gscore, int *max_off);
+/* Number of contexts the scalar drop-in's pool has created so far. */
+int64_t bsx_dropin_contexts(void);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,42 @@ def test_scalar_ksw_extend2_drop_in(engine):
         assert (s, *list(o)) == want
 
 
+def test_scalar_drop_in_from_short_lived_threads(engine):
+    """The reference's kt_for starts fresh worker threads for every batch of
+    reads (kthread.c:49-58); the drop-in's contexts must stay bounded by the
+    peak number of concurrent callers, not grow with the threads."""
+    import threading
+    from bsbolt_amd import bsx
+    L = bsx.lib()
+    mat = O.fill_scmat(1, 4)
+    rng = np.random.default_rng(81)
+    cases = []
+    for _ in range(64):
+        ql = int(rng.integers(1, 100)); q = rng.integers(0, 5, ql).astype(np.uint8)
+        t = np.concatenate([q % 4, rng.integers(0, 4, 30)]).astype(np.uint8)
+        h0 = int(rng.integers(1, 60))
+        cases.append((q, t, h0, O.extend2(q, t, h0, 100, 30, 100)[:6]))
+    bad = []
+
+    def worker(k):
+        q, t, h0, want = cases[k]
+        o = (ctypes.c_int * 5)()
+        s = L.bsx_ksw_extend2(len(q), q.ctypes.data, len(t), t.ctypes.data, 5, mat.ctypes.data,
+                              6, 1, 6, 1, 100, 30, 100, h0, *[ctypes.addressof(o) + 4 * j for j in range(5)])
+        if (s, *list(o)) != want:
+            bad.append(k)
+
+    conc = 4
+    for wave in range(0, len(cases), conc):  # 16 batches of 4 fresh threads each
+        th = [threading.Thread(target=worker, args=(k,)) for k in range(wave, wave + conc)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+    assert not bad, bad
+    assert 1 <= L.bsx_dropin_contexts() <= conc + 1  # +1: the single-thread test before may have made one
+
+
 def test_extend_is_deterministic(engine):
     from bsbolt_amd.tasks import random_ext_tasks
     t = random_ext_tasks(20000, seed=33)
