@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "libbsx.so")
 
 BSX_OK, BSX_EINVAL, BSX_EDEVICE, BSX_ENOMEM, BSX_ERANGE = 0, -1, -2, -3, -4
 BSX_MAX_QLEN = 2048
+BSX_NOALLOC_MAX_TASKS = 262144
 _ERRNAME = {BSX_EINVAL: "BSX_EINVAL", BSX_EDEVICE: "BSX_EDEVICE", BSX_ENOMEM: "BSX_ENOMEM",
             BSX_ERANGE: "BSX_ERANGE"}
 
@@ -35,7 +36,8 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_labels_device", "bsx_band_forward_device", "bsx_band_grad_device",
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
-           "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device", "bsx_dropin_contexts")
+           "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device", "bsx_dropin_contexts",
+           "bsx_extend_batch_device_noalloc")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -127,6 +129,7 @@ def lib():
         L.bsx_default_scoring.argtypes = [C.POINTER(Scoring)]
         for f, tk, rs in (("bsx_extend_batch", _ExtTasks, _ExtResults),
                           ("bsx_extend_batch_device", _ExtTasks, _ExtResults),
+                          ("bsx_extend_batch_device_noalloc", _ExtTasks, _ExtResults),
                           ("bsx_global_batch", _GlbTasks, _GlbResults),
                           ("bsx_global_batch_device", _GlbTasks, _GlbResults)):
             getattr(L, f).argtypes = [C.c_void_p, C.POINTER(Scoring), C.POINTER(tk), C.POINTER(rs)]
@@ -281,16 +284,18 @@ class Engine:
             self._check(rc)
         return res if raise_on_error else (res, rc)
 
-    def extend_device(self, dev, scoring: Scoring = None):
+    def extend_device(self, dev, scoring: Scoring = None, noalloc: bool = False):
         """Device-memory batch: `dev` maps the task/result field names to
         device addresses (e.g. torch tensors' data_ptr()).  Asynchronous; call
-        sync() to wait and collect precondition errors."""
+        sync() to wait and collect precondition errors.  noalloc=True calls the
+        allocation-free entry point (at most BSX_NOALLOC_MAX_TASKS tasks)."""
         sc = scoring or Scoring.make()
         tk = _ExtTasks(dev["n"], *[dev.get(k) for k in
                                    ("query", "q_off", "qlen", "target", "t_off", "tlen", "h0", "w",
                                     "end_bonus")])
         rs = _ExtResults(*[dev.get(k) for k in RESULT_FIELDS + ("cells",)])
-        self._check(self.L.bsx_extend_batch_device(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
+        f = self.L.bsx_extend_batch_device_noalloc if noalloc else self.L.bsx_extend_batch_device
+        self._check(f(self.ctx, C.byref(sc), C.byref(tk), C.byref(rs)))
 
     # ---- graph-captured submission ----
     def capture_extend(self, dev, scoring: Scoring = None):

@@ -95,6 +95,7 @@ struct bsx_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool pending_check = false;
   int pending_kind = 0;  // 1 = extension, 2 = global
+  void* scratch = nullptr;  // this device's scratch array (allocation-free entry point)
 };
 
 static int fail(bsx_ctx* c, int code, const char* fmt, ...) {
@@ -145,6 +146,33 @@ static int mat_max(const bsx_scoring* sc) {
   return m;
 }
 
+// ---- allocation-free extension entry point ----
+// Workspace of bsx_extend_batch_device_noalloc: one file-scope device array
+// per GPU, sized for BSX_NOALLOC_MAX_TASKS tasks and held until the process
+// ends.  Host code never uses the array's name as an address (on the host it
+// is a stand-in, not device memory): bsx_create looks up each device's
+// address once with hipGetSymbolAddress and every use goes through it.
+static constexpr size_t kNoallocN = (size_t)BSX_NOALLOC_MAX_TASKS + 1;
+static constexpr size_t kScratchBytes = 4 * align_up(kNoallocN) +        // bin_of, nbin_of, need_plain, cls
+                                        align_up(sizeof(ExtDesc) * kNoallocN) + align_up(32 * kNoallocN) +
+                                        align_up(kSmallBytes);
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's scratch array (the device must be current);
+// nullptr if the lookup failed.
+static void* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
 extern "C" {
 
 const char* bsx_version(void) { return "bsx 0.1 (gfx950)"; }
@@ -191,6 +219,7 @@ bsx_ctx* bsx_create(int device) {
   c->err_host[0] = 0;
   c->err_host[1] = INT_MAX;
   c->err_host[2] = 0;
+  c->scratch = scratch_address(device);  // once per device; nullptr leaves only the no-alloc call failing
   // work granularity of the extension kernels (tuning knob; defaults in bsx_internal.h)
   const char* tp = getenv("BSX_TPB_PLAIN");
   const char* tn = getenv("BSX_TPB_NARROW");
@@ -309,6 +338,36 @@ static int ext_buffers(bsx_ctx* c, size_t n) {
   return BSX_OK;
 }
 
+// Queues one extension batch on the context's stream with the workspace `wk`
+// (ExtWork arrays sized for the batch) and `cls` ([n] predicted classes, used
+// under the learned policy).
+static int extend_queue(bsx_ctx* c, const ExtArgs& a, ExtWork wk, uint8_t* cls) {
+  const size_t n = (size_t)a.n;
+  if (c->kt_used < c->kt_calls) {  // per-kernel timing of this call
+    wk.tev = &c->kt_ev[(size_t)c->kt_used * 2 * kTimedSlots];
+    wk.tmask = &c->kt_mask[c->kt_used];
+    *wk.tmask = 0;
+    ++c->kt_used;
+  }
+  BandPlan plan{kBandPlain, 0, nullptr, false, false};
+  if (c->band_policy == BSX_BAND_LEARNED) {
+    if (!c->model_ready) return fail(c, BSX_EINVAL, "learned band policy without a band model (bsx_band_model_set)");
+    time_begin(wk, kTimedSlots - 1, c->stream);
+    HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), cls, c->num_cu, c->stream), "band predict");
+    time_end(wk, kTimedSlots - 1, c->stream);
+    plan = BandPlan{kBandLearned, 0, cls, false, false};
+  } else if (c->band_policy == BSX_BAND_FIXED) {
+    plan = BandPlan{kBandFixed, c->band_class, nullptr, true, false};
+  }
+  HIPCHK(c, launch_extend(a, wk, c->num_cu, c->stream, nullptr, &plan), "extension launch");
+  HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+         "hipMemcpyAsync(err)");
+  c->last_n = plan.mode == kBandPlain ? 0 : (int64_t)n;
+  c->pending_check = true;
+  c->pending_kind = 1;
+  return BSX_OK;
+}
+
 int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t,
                             bsx_ext_results* r) {
   if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
@@ -321,31 +380,43 @@ int bsx_extend_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tas
   const size_t n = (size_t)t->n;
   rc = ext_buffers(c, n);
   if (rc) return rc;
+  if (c->band_policy == BSX_BAND_LEARNED) HIPCHK(c, c->ws->cls.ensure(n + 1), "hipMalloc(cls)");
+  return extend_queue(c, a, ext_work(c), (uint8_t*)c->ws->cls.p);
+}
+
+int bsx_extend_batch_device_noalloc(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t,
+                                    bsx_ext_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  if (t->n > BSX_NOALLOC_MAX_TASKS)
+    return fail(c, BSX_EINVAL, "bsx_extend_batch_device_noalloc takes at most %d tasks (got %lld)",
+                BSX_NOALLOC_MAX_TASKS, (long long)t->n);
+  if (!c->scratch) return fail(c, BSX_EDEVICE, "no scratch array on device %d (hipGetSymbolAddress failed)", c->device);
+  ExtArgs a;
+  int rc = ext_args(c, sc, t, a);
+  if (rc) return rc;
+  a.score = r->score; a.qle = r->qle; a.tle = r->tle; a.gtle = r->gtle; a.gscore = r->gscore;
+  a.max_off = r->max_off; a.cells = r->cells;
+  (void)hipSetDevice(c->device);
+  // carve the workspace out of the scratch array (layout of kScratchBytes);
+  // every counter in it is cleared by the first kernel of the launch sequence
+  char* b = (char*)c->scratch;
   ExtWork wk = ext_work(c);
-  if (c->kt_used < c->kt_calls) {  // per-kernel timing of this call
-    wk.tev = &c->kt_ev[(size_t)c->kt_used * 2 * kTimedSlots];
-    wk.tmask = &c->kt_mask[c->kt_used];
-    *wk.tmask = 0;
-    ++c->kt_used;
-  }
-  BandPlan plan{kBandPlain, 0, nullptr, false, false};
-  if (c->band_policy == BSX_BAND_LEARNED) {
-    if (!c->model_ready) return fail(c, BSX_EINVAL, "learned band policy without a band model (bsx_band_model_set)");
-    HIPCHK(c, c->ws->cls.ensure(n + 1), "hipMalloc(cls)");
-    time_begin(wk, kTimedSlots - 1, c->stream);
-    HIPCHK(c, mlp_launch_predict_tasks(a, model_view(c), (uint8_t*)c->ws->cls.p, c->num_cu, c->stream), "band predict");
-    time_end(wk, kTimedSlots - 1, c->stream);
-    plan = BandPlan{kBandLearned, 0, (const uint8_t*)c->ws->cls.p, false, false};
-  } else if (c->band_policy == BSX_BAND_FIXED) {
-    plan = BandPlan{kBandFixed, c->band_class, nullptr, true, false};
-  }
-  HIPCHK(c, launch_extend(a, wk, c->num_cu, c->stream, nullptr, &plan), "extension launch");
-  HIPCHK(c, hipMemcpyAsync(c->err_host, wk.err, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
-         "hipMemcpyAsync(err)");
-  c->last_n = plan.mode == kBandPlain ? 0 : (int64_t)n;
-  c->pending_check = true;
-  c->pending_kind = 1;
-  return BSX_OK;
+  wk.bin_of = (uint8_t*)b; b += align_up(kNoallocN);
+  wk.nbin_of = (uint8_t*)b; b += align_up(kNoallocN);
+  wk.need_plain = (uint8_t*)b; b += align_up(kNoallocN);
+  uint8_t* cls = (uint8_t*)b; b += align_up(kNoallocN);
+  wk.desc = (ExtDesc*)b; b += align_up(sizeof(ExtDesc) * kNoallocN);
+  wk.rec = (int32_t*)b; b += align_up(32 * kNoallocN);
+  wk.counts = (uint32_t*)b;
+  wk.cursor = wk.counts + kCountSlots;
+  wk.err = (int32_t*)(wk.cursor + kCountSlots);
+  wk.fetch = (uint32_t*)(wk.err + 4);
+  wk.split = wk.fetch + kFetchSlots;
+  wk.bcount = wk.split + 2 * kLaneFrameClasses;
+  wk.bcursor = wk.bcount + kExtBins * kRowBuckets;
+  wk.fcount = wk.bcursor + kExtBins * kRowBuckets;
+  wk.fcursor = wk.fcount + kLaneFrameClasses * kRowBuckets;
+  return extend_queue(c, a, wk, cls);
 }
 
 int bsx_set_side_streams(bsx_ctx* c, int enable) {

@@ -128,6 +128,21 @@ int bsx_extend_batch(bsx_ctx *ctx, const bsx_scoring *sc,
 int bsx_extend_batch_device(bsx_ctx *ctx, const bsx_scoring *sc,
                             const bsx_ext_tasks *tasks, bsx_ext_results *res);
 
+/* Allocation-free form of bsx_extend_batch_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and policies (plain, learned,
+ * fixed frame) as bsx_extend_batch_device, but the workspace is a static
+ * device array of about 84 bytes per task for BSX_NOALLOC_MAX_TASKS tasks
+ * (about 22 MB), which each GPU the library runs on holds until the process
+ * ends; the call itself allocates nothing.  More than BSX_NOALLOC_MAX_TASKS
+ * tasks -> BSX_EINVAL.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (two contexts, or two streams) race and must be
+ * ordered by the caller. */
+#define BSX_NOALLOC_MAX_TASKS 262144
+int bsx_extend_batch_device_noalloc(bsx_ctx *ctx, const bsx_scoring *sc,
+                                    const bsx_ext_tasks *tasks, bsx_ext_results *res);
+
 /* ---- global alignment + CIGAR (ksw_global2) ---- */
 int bsx_global_batch(bsx_ctx *ctx, const bsx_scoring *sc,
                      const bsx_glb_tasks *tasks, bsx_glb_results *res);
