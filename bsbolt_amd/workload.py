@@ -201,14 +201,26 @@ class Workload:
         n = wl().wl_regions(self.ctx, i, _p(out), cap)
         return out[:min(n, cap)]
 
-    def build_ext_batch(self, extend, max_rounds=64, on_round=None):
+    def build_ext_batch(self, extend, max_rounds=64, on_round=None, seeds=None):
         """Runs mem_chain2aln over every read in rounds; `extend(ExtTasks) ->
         dict of result arrays`.  Returns (all tasks issued as one ExtTasks,
-        their results, per-task side 0=left/1=right, per-task round)."""
+        their results, per-task side 0=left/1=right, per-task round); the read
+        of each task is left in self.task_read.  `seeds` = (read, chain, rbeg,
+        qbeg, len) arrays replaces the simulator's seeds (e.g. chains taken
+        from the reference aligner's own trace); a seed's score is its length,
+        as mem_chain sets it for SMEM seeds."""
         L = wl()
-        L.wl_chains_init(self.ctx, len(self.seed_read), _p(self.seed_read), _p(self.seed_chain), _p(self.seed_rbeg),
-                         _p(self.seed_qbeg), _p(self.seed_len))
-        parts, results, sides, rounds = [], [], [], []
+        sr, sc, sb, sq, sl = (self.seed_read, self.seed_chain, self.seed_rbeg, self.seed_qbeg, self.seed_len) \
+            if seeds is None else (np.ascontiguousarray(seeds[0], np.int32), np.ascontiguousarray(seeds[1], np.int32),
+                                   np.ascontiguousarray(seeds[2], np.int64), np.ascontiguousarray(seeds[3], np.int32),
+                                   np.ascontiguousarray(seeds[4], np.int32))
+        nch = L.wl_chains_init(self.ctx, len(sr), _p(sr), _p(sc), _p(sb), _p(sq), _p(sl))
+        # read of each chain: chains are consecutive runs of equal (read, chain) keys
+        key_change = np.ones(len(sr), bool)
+        key_change[1:] = (sr[1:] != sr[:-1]) | (sc[1:] != sc[:-1])
+        chain_read = sr[key_change]
+        assert len(chain_read) == nch
+        parts, results, sides, rounds, treads = [], [], [], [], []
         for r in range(max_rounds):
             n = L.wl_round_emit(self.ctx)
             if n == 0:
@@ -225,12 +237,13 @@ class Workload:
             res = extend(tasks)
             L.wl_round_consume(self.ctx, *[_p(np.ascontiguousarray(res[k], np.int32)) for k in RESULT_FIELDS])
             parts.append(tasks); results.append({k: np.asarray(res[k]) for k in RESULT_FIELDS})
-            sides.append(side); rounds.append(np.full(n, r, np.int32))
+            sides.append(side); rounds.append(np.full(n, r, np.int32)); treads.append(chain_read[cid])
             if on_round:
                 on_round(r, n)
         else:
             raise RuntimeError("mem_chain2aln rounds did not converge")
         batch = concat_ext(parts)
+        self.task_read = np.concatenate(treads) if treads else np.zeros(0, np.int32)
         res = {k: np.concatenate([x[k] for x in results]) if results else np.zeros(0, np.int32)
                for k in RESULT_FIELDS}
         return batch, res, (np.concatenate(sides) if sides else np.zeros(0, np.int32)), \
@@ -260,21 +273,10 @@ def cigar_band(w_, l_query, rlen, mat0=1, o_del=6, e_del=1, o_ins=6, e_ins=1):
 
 
 def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only=True, return_regions=False):
-    """CIGAR pass over every alignment region of a workload, in rounds, as
-    mem_reg2aln (bwamem.c:1197-1233) and bwa_gen_cigar2 (bwa.c:199-248) run it:
-    band w2 = max of the inferred deletion/insertion bands (capped by the
-    region's band when above -w), then up to 3 tries doubling w2 (at most 4w)
-    while the global score stays below truesc - a and changes.  Each try of a
-    region reverses query and reference when rb >= l_pac (leftmost indels) and
-    runs ksw_global2 at the band bwa_gen_cigar2 derives from w2; a region with
-    w2 == 0 and equal lengths is scored ungapped without DP (bwa.c:222-230).
-
-    `global_fn(GlbTasks) -> (score, n_cigar, cigar[n, slot])` runs one round.
-    Returns (all issued tasks as one GlbTasks, their scores, n_cigar, CIGAR
-    lists, region index per task, round per task, final score per region), and
-    with return_regions a dict of per-region arrays read, qb, qe, rb, re."""
-    o = dict(DEFAULT_OPT) if opt is None else opt
-    mat = fill_matrix(o["a"], o["b"]) if mat is None else np.asarray(mat, np.int8).reshape(25)
+    """CIGAR pass over every alignment region of a workload (see cigar_tries),
+    in rounds.  primary_only keeps each read's best region, the one its SAM
+    record comes from.  Returns what cigar_tries returns; with return_regions
+    also a dict of per-region arrays read, qb, qe, rb, re."""
     regs = []
     for i in range(wl.n_reads):
         rs_ = wl.regions(i)
@@ -285,6 +287,34 @@ def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only
             if score < min_score or qe <= qb or rb >= re:
                 continue
             regs.append((i, qb, qe, rb, re, truesc, w))
+    out = cigar_tries(regs, wl.read, wl.get_seq, wl.l_pac, global_fn, mat=mat, opt=opt)
+    if return_regions:
+        R = np.array([r for r in regs], np.int64).reshape(-1, 7)
+        reg = {"read": R[:, 0].copy()}
+        for k, c in (("qb", 1), ("qe", 2), ("rb", 3), ("re", 4)):
+            reg[k] = R[:, c].copy()
+        out = out + (reg,)
+    return out
+
+
+def cigar_tries(regs, read_fn, get_seq, l_pac, global_fn, mat=None, opt=None, trace=None):
+    """The CIGAR tries of mem_reg2aln (bwamem.c:1197-1233) and bwa_gen_cigar2
+    (bwa.c:199-248) for a list of regions (read, qb, qe, rb, re, truesc, w),
+    run in rounds: band w2 = max of the inferred deletion/insertion bands
+    (capped by the region's band when above -w), then up to 3 tries doubling w2
+    (at most 4w) while the global score stays below truesc - a and changes.
+    Each try reverses query and reference when rb >= l_pac (leftmost indels)
+    and runs ksw_global2 at the band bwa_gen_cigar2 derives from w2; a region
+    with w2 == 0 and equal lengths is scored ungapped without DP (bwa.c:222-230).
+
+    read_fn(i) gives read i's converted codes, get_seq(rb, re) the reference
+    window; `global_fn(GlbTasks) -> (score, n_cigar, cigar[n, slot])` runs one
+    round.  Returns (all issued tasks as one GlbTasks, their scores, n_cigar,
+    CIGAR lists, region index per task, round per task, final score per
+    region).  A `trace` list receives (region, try, w2 passed to
+    bwa_gen_cigar2, score) for every try, in the order of the tries."""
+    o = dict(DEFAULT_OPT) if opt is None else opt
+    mat = fill_matrix(o["a"], o["b"]) if mat is None else np.asarray(mat, np.int8).reshape(25)
     nreg = len(regs)
     R = np.array([r[1:] for r in regs], np.int64).reshape(-1, 6)
     qlen = R[:, 1] - R[:, 0]
@@ -309,12 +339,12 @@ def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only
         for k, g in enumerate(idx):
             i, qb, qe, rb, re = regs[g][:5]
             if g not in seqs:
-                qs = wl.read(i)[qb:qe].copy()
-                rs = wl.get_seq(rb, re)
-                if len(rs) != re - rb or (rb < wl.l_pac < re):
+                qs = read_fn(i)[qb:qe].copy()
+                rs = get_seq(rb, re)
+                if len(rs) != re - rb or (rb < l_pac < re):
                     seqs[g] = None
                 else:
-                    if rb >= wl.l_pac:  # leftmost indels on the reverse strand (bwa.c:216-221)
+                    if rb >= l_pac:  # leftmost indels on the reverse strand (bwa.c:216-221)
                         qs, rs = qs[::-1].copy(), rs[::-1].copy()
                     seqs[g] = (qs, rs)
             if seqs[g] is None:
@@ -340,6 +370,8 @@ def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only
             owner.append(idx[np.asarray(pidx, np.int64)])
             rnd.append(np.full(len(pairs), rr, np.int32))
         # do-while tail of mem_reg2aln (bwamem.c:1229-1233)
+        if trace is not None:
+            trace.extend((int(g), rr, int(w2[g]), int(sc_round[k])) for k, g in enumerate(idx))
         final[idx] = sc_round
         stop = (sc_round == last[idx]) | (w2[idx] == wmax)
         cont = ~stop
@@ -351,14 +383,8 @@ def build_glb_batch(wl, global_fn, min_score=0, mat=None, opt=None, primary_only
         active[idx[cont]] = True
     batch = concat_glb(parts)
     cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
-    out = (batch, cat(scores, np.int32), cat(ncig, np.int32), cigs, cat(owner, np.int64), cat(rnd, np.int32),
-           final)
-    if return_regions:
-        reg = {"read": np.array([r[0] for r in regs], np.int64)}
-        for k, c in (("qb", 0), ("qe", 1), ("rb", 2), ("re", 3)):
-            reg[k] = R[:, c].copy()
-        out = out + (reg,)
-    return out
+    return (batch, cat(scores, np.int32), cat(ncig, np.int32), cigs, cat(owner, np.int64), cat(rnd, np.int32),
+            final)
 
 
 def md_inputs(wl, regions, owner, cigars):

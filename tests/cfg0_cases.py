@@ -1,0 +1,96 @@
+"""The cfg0 batch as the reference aligner issued it (tests/golden/cfg0_bwamem.npz,
+written by tests/golden/make_golden_cfg0.py from bsbolt/External/BWA compiled
+here), and replays of its chains and regions through this project's task
+producer, shared by the CPU and GPU tests."""
+import numpy as np
+
+from conftest import load_golden
+
+FIELDS = ("score", "qle", "tle", "gtle", "gscore", "max_off")
+_FX = None
+
+
+def fixture():
+    global _FX
+    if _FX is None:
+        _FX = load_golden("cfg0_bwamem.npz")
+    return _FX
+
+
+def ext_tasks():
+    """Every ksw_extend2 call of bwa mem on cfg0: (ExtTasks, scoring dict, reference outputs)."""
+    from bsbolt_amd.tasks import ExtTasks, EXT_FIELDS
+    z = fixture()
+    t = ExtTasks(*[z[k] for k in EXT_FIELDS])
+    sc = dict(mat=z["mat"], o_del=int(z["o_del"]), e_del=int(z["e_del"]), o_ins=int(z["o_ins"]),
+              e_ins=int(z["e_ins"]), zdrop=int(z["zdrop"]))
+    return t, sc, {k: z["ref_" + k] for k in FIELDS}
+
+
+def workload():
+    from bsbolt_amd.workload import CONFIGS, Workload
+    return Workload(CONFIGS["cfg0"])
+
+
+def replay_chains(wl, extend):
+    """The reference's chains of the traced reads through the producer's
+    mem_chain2aln rounds.  Returns (tasks in the reference's call order, their
+    results, read of each task, regions per read as (qb, qe, rb, re, score, w))."""
+    z = fixture()
+    nt = int(z["n_trace"])
+    batch, res, side, rnd = wl.build_ext_batch(
+        extend, seeds=(z["seed_read"], z["seed_chain"], z["seed_rbeg"], z["seed_qbeg"], z["seed_len"]))
+    order = np.lexsort((rnd, wl.task_read))  # per read, round order = the reference's sequential order
+    regions = {i: [tuple(int(x) for x in r[[0, 1, 2, 3, 4, 6]]) for r in wl.regions(i)] for i in range(nt)}
+    return batch, res, wl.task_read[order], order, regions
+
+
+def reference_calls_of_traced_reads():
+    """Indices (into the fixture's call list) of the calls of the traced reads."""
+    z = fixture()
+    return np.nonzero(z["e_read"] < int(z["n_trace"]))[0]
+
+
+def reg2aln_regions():
+    """Each mem_reg2aln call of the traced reads as a region (read, qb, qe, rb,
+    re, truesc, w) -- qb/qe from the read's final region with that window --
+    with the reference's try: w2, global score, CIGAR, NM and MD/XB text."""
+    z = fixture()
+    R = z["reg2aln"]
+    fin = z["final_region"]
+    tails = bytes(z["reg2aln_tail"]).decode().split("\n")
+    cig_off = np.concatenate([[0], np.cumsum(R[:, 12])])
+    out = []
+    for k, row in enumerate(R):
+        i, ci, ti, inf, cmd, alnreg, w2, gsc, lsc, lq, rb, re, nc, nm = (int(x) for x in row[:14])
+        m = fin[(fin[:, 0] == i) & (fin[:, 4] == rb) & (fin[:, 5] == re) & (fin[:, 3] - fin[:, 2] == lq)]
+        assert len(m) >= 1, f"no final region of read {i} for window [{rb},{re})"
+        qb, qe = int(m[0, 2]), int(m[0, 3])
+        md, _, xb = tails[k].partition("\tXB:Z:")
+        out.append(dict(region=(i, qb, qe, rb, re, lsc, alnreg), w2=w2, score=gsc, n_cigar=nc, nm=nm,
+                        cigar=z["reg2aln_cigar"][cig_off[k]:cig_off[k] + nc], md=md, xb=xb,
+                        meth=[int(x) for x in row[14:18]], inferred=inf))
+    return out
+
+
+def check_replay(batch, res, task_read, order, regions):
+    """The replayed producer issues exactly the reference's calls, and ends
+    with exactly its regions."""
+    z = fixture()
+    idx = reference_calls_of_traced_reads()
+    assert len(order) == len(idx), f"producer issued {len(order)} calls, the reference {len(idx)}"
+    assert np.array_equal(task_read, z["e_read"][idx])
+    for k in ("qlen", "tlen", "h0", "w", "end_bonus"):
+        got = getattr(batch, k)[order]
+        assert np.array_equal(got, z[k][idx]), f"task field {k} differs from the reference's calls"
+    for j, (a, b) in enumerate(zip(order, idx)):
+        qo, to = int(z["q_off"][b]), int(z["t_off"][b])
+        assert np.array_equal(batch.query_of(a), z["query"][qo:qo + int(z["qlen"][b])]), f"call {j}: query"
+        assert np.array_equal(batch.target_of(a), z["target"][to:to + int(z["tlen"][b])]), f"call {j}: target"
+    for k in FIELDS:
+        assert np.array_equal(np.asarray(res[k])[order], z["ref_" + k][idx]), f"result {k}"
+    reg = z["region"]
+    for i in range(int(z["n_trace"])):
+        want = [(int(r[1]), int(r[2]), int(r[3]), int(r[4]), int(r[5]), max(int(r[6]), int(r[7])))
+                for r in reg[reg[:, 0] == i]]
+        assert regions[i] == want, f"read {i}: regions {regions[i]} != reference {want}"
