@@ -37,7 +37,7 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
            "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device", "bsx_dropin_contexts",
-           "bsx_extend_batch_device_noalloc")
+           "bsx_extend_batch_device_noalloc", "bsx_ksw_global2")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -177,6 +177,8 @@ def lib():
             getattr(L, f).restype = C.c_int
         L.bsx_ksw_extend2.restype = C.c_int
         L.bsx_ksw_extend2.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P] + [C.c_int] * 8 + [P] * 5
+        L.bsx_ksw_global2.restype = C.c_int
+        L.bsx_ksw_global2.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P] + [C.c_int] * 5 + [P, P]
         L.bsx_dropin_contexts.restype = C.c_int64
         L.bsx_dropin_contexts.argtypes = []
         _lib = L

@@ -1041,4 +1041,33 @@ int bsx_ksw_extend2(int qlen, const uint8_t* query, int tlen, const uint8_t* tar
   return o[0];
 }
 
+int bsx_ksw_global2(int qlen, const uint8_t* query, int tlen, const uint8_t* target, int m, const int8_t* mat,
+                    int o_del, int e_del, int o_ins, int e_ins, int w, int* n_cigar, uint32_t** cigar) {
+  if (n_cigar) *n_cigar = 0;  // ksw.c:510
+  if (m != 5 || !query || !target || !mat || qlen < 1 || tlen < 1) return INT32_MIN;
+  const bool want = n_cigar && cigar;
+  bsx_ctx* ctx = dropin_acquire();
+  if (!ctx) return INT32_MIN;
+  bsx_scoring sc;
+  memcpy(sc.mat, mat, 25);
+  sc.o_del = o_del; sc.e_del = e_del; sc.o_ins = o_ins; sc.e_ins = e_ins; sc.zdrop = 0;
+  const uint64_t zero = 0;
+  bsx_glb_tasks t{1, query, &zero, &qlen, target, &zero, &tlen, &w};
+  const int64_t slot = (int64_t)qlen + tlen;  // a CIGAR has at most qlen + tlen ops
+  std::vector<uint32_t> buf(want ? (size_t)slot : 0);
+  int32_t score = 0, nc = 0;
+  bsx_glb_results r{&score, &nc, want ? buf.data() : nullptr, want ? slot : 0};
+  const int rc = bsx_global_batch(ctx, &sc, &t, &r);
+  dropin_release(ctx);
+  if (rc != BSX_OK || nc < 0) return INT32_MIN;
+  if (want) {  // a fresh malloc'd CIGAR the caller frees, as push_cigar leaves it (ksw.c:492-502,602)
+    uint32_t* c = nc > 0 ? (uint32_t*)malloc(4 * (size_t)nc) : nullptr;
+    if (nc > 0 && !c) return INT32_MIN;
+    if (nc > 0) memcpy(c, buf.data(), 4 * (size_t)nc);
+    *n_cigar = nc;
+    *cigar = c;
+  }
+  return score;
+}
+
 }  // extern "C"

@@ -316,7 +316,20 @@ int bsx_ksw_extend2(int qlen, const uint8_t *query, int tlen, const uint8_t *tar
                     int m, const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins,
                     int w, int end_bonus, int zdrop, int h0,
                     int *qle, int *tle, int *gtle, int *gscore, int *max_off);
-/* Number of contexts the scalar drop-in's pool has created so far. */
+/* Scalar drop-in with the exact signature of ksw_global2 (ksw.h:84), run as a
+ * one-task batch on a pooled context like bsx_ksw_extend2.  With n_cigar and
+ * cigar non-NULL, *cigar receives a fresh malloc'd BAM CIGAR of *n_cigar ops
+ * that the caller frees (ksw.h:79; the reference builds it with realloc from
+ * NULL, ksw.c:492-502,602); with either NULL only the score is computed.  It
+ * returns INT32_MIN (and *n_cigar = 0, *cigar untouched) where a batch task
+ * would be rejected: m != 5, qlen or tlen < 1 or > 65536, |tlen - qlen| > w
+ * (the reference reads uninitialised memory there), min(qlen, 2w+1) > 2048 or
+ * min(qlen, 2w+1)*tlen above the context's direction slot; and on a device
+ * error.  bwa_gen_cigar2 (bwa.c:231-248) always passes w >= |tlen - qlen| + 3. */
+int bsx_ksw_global2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int m,
+                    const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins, int w,
+                    int *n_cigar, uint32_t **cigar);
+/* Number of contexts the scalar drop-ins' pool has created so far. */
 int64_t bsx_dropin_contexts(void);
 
 #ifdef __cplusplus

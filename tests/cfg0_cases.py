@@ -94,3 +94,28 @@ def check_replay(batch, res, task_read, order, regions):
         want = [(int(r[1]), int(r[2]), int(r[3]), int(r[4]), int(r[5]), max(int(r[6]), int(r[7])))
                 for r in reg[reg[:, 0] == i]]
         assert regions[i] == want, f"read {i}: regions {regions[i]} != reference {want}"
+
+
+# BSBolt Align's bwa mem argv at its defaults (bsbolt/Utils/Launcher.py:77-108;
+# bsbolt/Utils/Parser.py:31-115), one thread
+BSB_ARGV = ["-Y", "-A", "1", "-B", "4", "-D", "0.5", "-E", "1,1", "-L", "30,30", "-T", "10", "-U", "17",
+            "-W", "0", "-c", "500", "-d", "100", "-k", "19", "-m", "50", "-r", "1.5", "-t", "1", "-w", "100",
+            "-y", "20", "-O", "6,6", "-h", "100,200", "-e", "0.1", "-l", "0.5", "-n", "5", "-Z", "0.95"]
+
+
+def write_fasta_fastq(d, wl, n_reads=None):
+    """The workload's genome as a one-contig FASTA and its reads as sequenced
+    as FASTQ (names r0, r1, ...); returns (fasta path, fastq path)."""
+    import os
+    acgtn = np.frombuffer(b"ACGTN", np.uint8)
+    fa, fq = os.path.join(d, "ref.fa"), os.path.join(d, "reads.fq")
+    with open(fa, "w") as f:
+        f.write(">chr1\n")
+        s = acgtn[wl.genome].tobytes().decode()
+        for i in range(0, len(s), 80):
+            f.write(s[i:i + 80] + "\n")
+    with open(fq, "w") as f:
+        for i in range(wl.n_reads if n_reads is None else min(n_reads, wl.n_reads)):
+            r = acgtn[wl.oread(i)].tobytes().decode()
+            f.write(f"@r{i}\n{r}\n+\n{'I' * len(r)}\n")
+    return fa, fq

@@ -149,3 +149,36 @@ def test_global_wide_bands_walk_paths():
     assert np.array_equal(ncig, on)
     for i in range(len(tk)):
         assert np.array_equal(cig[i, :ncig[i]], oc[i, :on[i]]), f"task {i}"
+
+
+def test_scalar_ksw_global2_drop_in(engine):
+    """bsx_ksw_global2 through the C symbol with the reference's exact
+    signature (ksw.h:84): score, n_cigar and a malloc'd CIGAR the caller frees."""
+    import ctypes
+    from bsbolt_amd import bsx
+    L = bsx.lib()
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    rng = np.random.default_rng(84)
+    mat = O.fill_scmat(1, 4)
+    for _ in range(40):
+        ql = int(rng.integers(1, 150))
+        q = rng.integers(0, 5, ql).astype(np.uint8)
+        t = (q % 4).copy()
+        if ql > 10:  # an indel and a substitution
+            k = int(rng.integers(1, ql - 1))
+            t = np.concatenate([t[:k], rng.integers(0, 4, int(rng.integers(1, 4))).astype(np.uint8), t[k:]])
+            t[0] = (t[0] + 1) % 4
+        w = max(abs(len(t) - ql) + 3, int(rng.integers(1, 40)))
+        want_s, want_c = O.global2(q, t, w)
+        want_n = len(want_c)
+        nc = ctypes.c_int(-7)
+        cp = ctypes.POINTER(ctypes.c_uint32)()
+        s = L.bsx_ksw_global2(ql, q.ctypes.data, len(t), t.ctypes.data, 5, mat.ctypes.data, 6, 1, 6, 1, w,
+                              ctypes.byref(nc), ctypes.byref(cp))
+        assert (s, nc.value) == (want_s, want_n)
+        assert [cp[k] for k in range(nc.value)] == list(want_c)
+        libc.free(ctypes.cast(cp, ctypes.c_void_p))
+        # score only (n_cigar / cigar NULL, ksw.c:510-513)
+        assert L.bsx_ksw_global2(ql, q.ctypes.data, len(t), t.ctypes.data, 5, mat.ctypes.data, 6, 1, 6, 1, w,
+                                 None, None) == want_s
