@@ -37,7 +37,7 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
            "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device", "bsx_dropin_contexts",
-           "bsx_extend_batch_device_noalloc", "bsx_ksw_global2")
+           "bsx_extend_batch_device_noalloc", "bsx_ksw_global2", "bsx_reg2aln_cigar_device")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -100,6 +100,17 @@ class _MdTasks(C.Structure):
     _fields_ = [("n", C.c_int64), ("l_pac", C.c_int64), ("crick_l", C.c_int64), ("opac", C.c_void_p),
                 ("rb", C.c_void_p), ("re", C.c_void_p), ("oquery", C.c_void_p), ("oq_off", C.c_void_p),
                 ("l_query", C.c_void_p), ("n_cigar", C.c_void_p), ("cigar", C.c_void_p), ("cigar_slot", C.c_int64)]
+
+
+class _Reg2alnTasks(C.Structure):
+    _fields_ = [("n", C.c_int64), ("l_pac", C.c_int64), ("rb", C.c_void_p), ("re", C.c_void_p),
+                ("qb", C.c_void_p), ("qe", C.c_void_p), ("l_seq", C.c_void_p), ("n_cigar", C.c_void_p),
+                ("cigar", C.c_void_p), ("cigar_slot", C.c_int64)]
+
+
+class _Reg2alnResults(C.Structure):
+    _fields_ = [("cigar", C.c_void_p), ("cigar_slot", C.c_int64), ("n_cigar", C.c_void_p), ("pos", C.c_void_p),
+                ("is_rev", C.c_void_p)]
 
 
 class _MdResults(C.Structure):
@@ -169,6 +180,8 @@ def lib():
         L.bsx_set_side_streams.argtypes = [P, C.c_int]
         L.bsx_md_batch_device.argtypes = [P, C.POINTER(_MdTasks), C.POINTER(_MdResults)]
         L.bsx_md_batch_device.restype = C.c_int
+        L.bsx_reg2aln_cigar_device.argtypes = [P, C.POINTER(_Reg2alnTasks), C.POINTER(_Reg2alnResults)]
+        L.bsx_reg2aln_cigar_device.restype = C.c_int
         for f in ("bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits", "bsx_set_side_streams"):
             getattr(L, f).restype = C.c_int
         for f in ("bsx_set_band_policy", "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get",
@@ -508,6 +521,12 @@ class Engine:
                         dev["nm"], dev["meth"], dev["status"])
         self._check(self.L.bsx_md_batch_device(self.ctx, C.byref(tk), C.byref(rs)))
 
+    def reg2aln_cigar(self, l_pac, rb, re, qb, qe, l_seq, n_cigar, cigar):
+        """SAM CIGAR and position of regions from their bwa_gen_cigar2 CIGARs
+        (mem_reg2aln's deletion squeeze and clipping): host arrays in, host
+        arrays out (cigar [n, slot+2], n_cigar, pos, is_rev)."""
+        return _reg2aln_cigar(self, l_pac, rb, re, qb, qe, l_seq, n_cigar, cigar)
+
     def md(self, opac, l_pac, crick_l, rb, re, oquery, oq_off, l_query, n_cigar, cigar, md_slot=None, xb_slot=None):
         """Host arrays in, host arrays out: NM, MD:Z and XB:Z of each task's
         CIGAR (cigar is [n, slot] uint32 as global_ returns it).  Returns a dict
@@ -536,6 +555,26 @@ class Engine:
         return {"md": [bytes(h["md"][i, :h["md_len"][i]]).decode() for i in range(n)],
                 "xb": [bytes(h["xb"][i, :h["xb_len"][i]]).decode() for i in range(n)],
                 "nm": h["nm"], "meth": h["meth"], "status": h["status"]}
+
+
+def _reg2aln_cigar(eng, l_pac, rb, re, qb, qe, l_seq, n_cigar, cigar):
+    n = len(rb)
+    cigar = np.ascontiguousarray(cigar, np.uint32).reshape(n, -1)
+    host = {"rb": np.ascontiguousarray(rb, np.int64), "re": np.ascontiguousarray(re, np.int64),
+            "qb": np.ascontiguousarray(qb, np.int32), "qe": np.ascontiguousarray(qe, np.int32),
+            "l_seq": np.ascontiguousarray(l_seq, np.int32), "n_cigar": np.ascontiguousarray(n_cigar, np.int32),
+            "cigar": cigar}
+    keep = {k: eng.to_device(v) for k, v in host.items()}
+    slot = cigar.shape[1] + 2
+    out = {"cigar": eng.empty((n, slot), np.uint32), "n_cigar": eng.empty((n,), np.int32),
+           "pos": eng.empty((n,), np.int64), "is_rev": eng.empty((n,), np.int32)}
+    tk = _Reg2alnTasks(n, int(l_pac), *[keep[k].ptr for k in ("rb", "re", "qb", "qe", "l_seq", "n_cigar", "cigar")],
+                       cigar.shape[1])
+    rs = _Reg2alnResults(out["cigar"].ptr, slot, out["n_cigar"].ptr, out["pos"].ptr, out["is_rev"].ptr)
+    eng._check(eng.L.bsx_reg2aln_cigar_device(eng.ctx, C.byref(tk), C.byref(rs)))
+    eng.sync()
+    h = {k: v.numpy() for k, v in out.items()}
+    return h["cigar"], h["n_cigar"], h["pos"], h["is_rev"]
 
 
 def _glb_subset(t, a, b):

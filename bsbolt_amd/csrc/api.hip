@@ -767,6 +767,24 @@ int bsx_md_batch_device(bsx_ctx* c, const bsx_md_tasks* t, bsx_md_results* r) {
   return BSX_OK;
 }
 
+int bsx_reg2aln_cigar_device(bsx_ctx* c, const bsx_reg2aln_tasks* t, bsx_reg2aln_results* r) {
+  if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");
+  if (t->n < 0 || t->n > INT_MAX) return fail(c, BSX_EINVAL, "n must be in [0, 2^31-1]");
+  if (t->n == 0) return BSX_OK;
+  if (!t->rb || !t->re || !t->qb || !t->qe || !t->l_seq || !t->n_cigar || !t->cigar || !r->cigar || !r->n_cigar ||
+      !r->pos || !r->is_rev)
+    return fail(c, BSX_EINVAL, "arrays must not be NULL");
+  if (t->l_pac < 1 || t->cigar_slot < 1 || r->cigar_slot < 1)
+    return fail(c, BSX_EINVAL, "need l_pac >= 1 and slots >= 1");
+  (void)hipSetDevice(c->device);
+  Reg2alnArgs a{};
+  a.n = t->n; a.l_pac = t->l_pac; a.rb = t->rb; a.re = t->re; a.qb = t->qb; a.qe = t->qe; a.l_seq = t->l_seq;
+  a.n_cigar = t->n_cigar; a.cigar = t->cigar; a.cigar_slot = t->cigar_slot;
+  a.out = r->cigar; a.out_slot = r->cigar_slot; a.out_n = r->n_cigar; a.pos = r->pos; a.is_rev = r->is_rev;
+  HIPCHK(c, launch_reg2aln_cigar(a, c->num_cu, c->stream), "reg2aln cigar launch");
+  return BSX_OK;
+}
+
 int bsx_global_batch_device(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t,
                             bsx_glb_results* r) {
   if (!c || !t || !r) return fail(c, BSX_EINVAL, "NULL argument");

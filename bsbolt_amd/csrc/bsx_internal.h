@@ -239,6 +239,22 @@ struct MdArgs {
 };
 hipError_t launch_md(const MdArgs& a, int num_cu, hipStream_t st);
 
+// SAM CIGAR and position of regions (mem_reg2aln's tail, bwamem.c:1237-1263);
+// fields as bsx_reg2aln_tasks / bsx_reg2aln_results in include/bsx.h.
+struct Reg2alnArgs {
+  int64_t n, l_pac;
+  const int64_t *rb, *re;
+  const int32_t *qb, *qe, *l_seq, *n_cigar;
+  const uint32_t* cigar;
+  int64_t cigar_slot;
+  uint32_t* out;
+  int64_t out_slot;
+  int32_t* out_n;
+  int64_t* pos;
+  int32_t* is_rev;
+};
+hipError_t launch_reg2aln_cigar(const Reg2alnArgs& a, int num_cu, hipStream_t st);
+
 // Blocks per CU of the global kernel by lanes per task (8-lane shapes hold
 // twice the groups per block, so fewer blocks keep the scratch the same
 // size); the scratch holds one slot per group.

@@ -238,4 +238,66 @@ hipError_t launch_md(const MdArgs& a, int num_cu, hipStream_t st) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// The tail of mem_reg2aln (bwamem.c:1237-1263): the SAM CIGAR of a region from
+// the CIGAR bwa_gen_cigar2 made for it.  The position is bns_depos of rb (or
+// of re - 1 on the reverse strand); a leading deletion is squeezed out and
+// moves the position, or else (only if the first op is not one) a trailing
+// deletion is dropped; then the unaligned read ends become clip ops
+// (len << 4 | 3), 5' first, swapped on the reverse strand.  One lane per task:
+// a handful of words each.
+__global__ void __launch_bounds__(256)
+reg2aln_cigar_kernel(Reg2alnArgs a) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t rb = a.rb[t], re = a.re[t];
+    const int qb = a.qb[t], qe = a.qe[t], lq = a.l_seq[t], nc = a.n_cigar[t];
+    const uint32_t* __restrict__ c = a.cigar + t * a.cigar_slot;
+    if (nc < 0 || nc > a.cigar_slot || rb < 0 || re <= rb || re > 2 * a.l_pac || qb < 0 || qe < qb || qe > lq) {
+      a.out_n[t] = -1;
+      a.pos[t] = -1;
+      a.is_rev[t] = 0;
+      continue;
+    }
+    const int64_t p0 = rb < a.l_pac ? rb : re - 1;  // bns_depos (bntseq.h)
+    const bool rev = p0 >= a.l_pac;
+    int64_t pos = rev ? 2 * a.l_pac - 1 - p0 : p0;
+    int first = 0, last = nc;
+    if (nc > 0) {
+      if ((c[0] & 0xf) == 2) {
+        pos += c[0] >> 4;
+        first = 1;
+      } else if ((c[nc - 1] & 0xf) == 2) {
+        last = nc - 1;
+      }
+    }
+    int clip5 = 0, clip3 = 0;
+    if (qb != 0 || qe != lq) {
+      clip5 = rev ? lq - qe : qb;
+      clip3 = rev ? qb : lq - qe;
+    }
+    const int n_out = (clip5 ? 1 : 0) + (last - first) + (clip3 ? 1 : 0);
+    a.pos[t] = pos;
+    a.is_rev[t] = rev ? 1 : 0;
+    if (n_out > a.out_slot) {
+      a.out_n[t] = -2;
+      continue;
+    }
+    uint32_t* __restrict__ o = a.out + t * a.out_slot;
+    int k = 0;
+    if (clip5) o[k++] = (uint32_t)clip5 << 4 | 3u;
+    for (int j = first; j < last; ++j) o[k++] = c[j];
+    if (clip3) o[k++] = (uint32_t)clip3 << 4 | 3u;
+    a.out_n[t] = k;
+  }
+}
+
+hipError_t launch_reg2aln_cigar(const Reg2alnArgs& a, int num_cu, hipStream_t st) {
+  if (a.n <= 0) return hipSuccess;
+  int64_t blocks = (a.n + 255) / 256;
+  if (blocks > (int64_t)num_cu * 16) blocks = (int64_t)num_cu * 16;
+  hipLaunchKernelGGL(reg2aln_cigar_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 }  // namespace bsx

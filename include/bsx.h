@@ -201,6 +201,36 @@ typedef struct {
 
 int bsx_md_batch_device(bsx_ctx *ctx, const bsx_md_tasks *tasks, bsx_md_results *res);
 
+/* ---- SAM CIGAR of a region (the tail of mem_reg2aln, bwamem.c:1237-1263) ----
+ * From the CIGAR bwa_gen_cigar2 made for region [rb, re) x [qb, qe) of a read
+ * of l_seq bases: the position bns_depos gives for rb (re - 1 on the reverse
+ * strand) and is_rev; a leading deletion is squeezed out and moves the
+ * position, or else (only if the first op is not a deletion) a trailing one is
+ * dropped; the unaligned ends become clip ops len << 4 | 3 (5' first, swapped
+ * on the reverse strand), which the SAM writer prints as S or H.  pos is on
+ * the doubled reference's forward copy (subtract the contig's offset for the
+ * SAM POS).  n_cigar_out[t] = -1 for a task outside its preconditions
+ * (n_cigar > cigar_slot, rb < 0, re <= rb, re > 2*l_pac, qb < 0, qe < qb,
+ * qe > l_seq), -2 when the result does not fit cigar_slot_out.  Device memory;
+ * queued on the context's stream. */
+typedef struct {
+	int64_t n, l_pac;
+	const int64_t  *rb, *re;
+	const int32_t  *qb, *qe, *l_seq, *n_cigar;
+	const uint32_t *cigar;
+	int64_t         cigar_slot;
+} bsx_reg2aln_tasks;
+
+typedef struct {
+	uint32_t *cigar;
+	int64_t   cigar_slot;
+	int32_t  *n_cigar;
+	int64_t  *pos;
+	int32_t  *is_rev;
+} bsx_reg2aln_results;
+
+int bsx_reg2aln_cigar_device(bsx_ctx *ctx, const bsx_reg2aln_tasks *tasks, bsx_reg2aln_results *res);
+
 /* ---- device memory and timing on the context's device/stream ----
  * Helpers for callers that hold their batches in device memory without a
  * HIP runtime of their own (e.g. Python through ctypes).  No counterpart in

@@ -15,6 +15,8 @@
  *   orc_nt4             bntseq.c:48-65         (nst_nt4_table)
  *   orc_gen_md          bwa.c:180-197,250-343  (NM, MD and XB:Z of bwa_gen_cigar2,
  *                                               getMethylationContext)
+ *   orc_reg2aln_cigar   bwamem.c:1237-1263     (SAM CIGAR and position in mem_reg2aln:
+ *                                               deletion squeeze, clipping)
  * Parity of each function is pinned by tests against golden vectors produced by
  * the compiled reference (oracle/_ref) and by closed-form cases.
  */
@@ -663,4 +665,37 @@ double orc_gen_md_batch(int64_t n, int64_t l_pac, const uint8_t *opac, int64_t c
 	double dt = now_s() - t0;
 	pthread_mutex_destroy(&J.mu);
 	return dt;
+}
+
+/* bwamem.c:1237-1263: the tail of mem_reg2aln.  From the CIGAR bwa_gen_cigar2
+ * returned for region [rb, re) x [qb, qe) of a read of l_seq bases: position
+ * = bns_depos(rb < l_pac ? rb : re - 1); a leading D is dropped and moves the
+ * position, else a trailing D is dropped; then clip ops (len<<4|3) for the
+ * unaligned read ends, 5' end first (qb forward, l_seq - qe reverse).  Writes
+ * *pos, *is_rev and out[]; returns the op count, -1 for inputs outside the
+ * reference's (n_cigar < 0, rb < 0, re <= rb, re > 2*l_pac, qb < 0, qe < qb,
+ * qe > l_seq), -2 when out_cap is too small. */
+int orc_reg2aln_cigar(int64_t l_pac, int64_t rb, int64_t re, int qb, int qe, int l_seq, int n_cigar,
+                      const uint32_t *cigar, uint32_t *out, int out_cap, int64_t *pos, int *is_rev)
+{
+	int64_t p;
+	int n = n_cigar, i, k = 0, clip5 = 0, clip3 = 0, skip_first = 0;
+	*pos = -1; *is_rev = 0;
+	if (n_cigar < 0 || rb < 0 || re <= rb || re > 2 * l_pac || qb < 0 || qe < qb || qe > l_seq) return -1;
+	p = rb < l_pac ? rb : re - 1;            /* bns_depos */
+	*is_rev = p >= l_pac;
+	*pos = *is_rev ? (l_pac << 1) - 1 - p : p;
+	if (n > 0) {                              /* squeeze out a leading or a trailing deletion */
+		if ((cigar[0] & 0xf) == 2) { *pos += cigar[0] >> 4; skip_first = 1; --n; }
+		else if ((cigar[n - 1] & 0xf) == 2) --n;
+	}
+	if (qb != 0 || qe != l_seq) {             /* clipping */
+		clip5 = *is_rev ? l_seq - qe : qb;
+		clip3 = *is_rev ? qb : l_seq - qe;
+	}
+	if ((clip5 > 0) + n + (clip3 > 0) > out_cap) return -2;
+	if (clip5) out[k++] = (uint32_t)clip5 << 4 | 3;
+	for (i = 0; i < n; ++i) out[k++] = cigar[skip_first + i];
+	if (clip3) out[k++] = (uint32_t)clip3 << 4 | 3;
+	return k;
 }

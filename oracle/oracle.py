@@ -282,3 +282,16 @@ def bs_convert(seq, pattern):
     buf = C.create_string_buffer(seq.encode())
     lib().orc_bs_convert(buf, pattern, len(seq))
     return buf.value.decode()
+
+
+def reg2aln_cigar(l_pac, rb, re, qb, qe, l_seq, cigar):
+    """orc_reg2aln_cigar for one region: (n_ops or -1/-2, [SAM CIGAR words], pos, is_rev)."""
+    L = lib()
+    c = np.ascontiguousarray(cigar, np.uint32)
+    out = np.zeros(len(c) + 2, np.uint32)
+    pos, rev = C.c_int64(0), C.c_int(0)
+    fn = L.orc_reg2aln_cigar
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P]
+    k = fn(l_pac, rb, re, qb, qe, l_seq, len(c), _ptr(c), _ptr(out), len(out), C.byref(pos), C.byref(rev))
+    return k, [int(x) for x in out[:max(k, 0)]], pos.value, rev.value

@@ -51,14 +51,15 @@ def reference_calls_of_traced_reads():
     return np.nonzero(z["e_read"] < int(z["n_trace"]))[0]
 
 
-def reg2aln_regions():
-    """Each mem_reg2aln call of the traced reads as a region (read, qb, qe, rb,
-    re, truesc, w) -- qb/qe from the read's final region with that window --
-    with the reference's try: w2, global score, CIGAR, NM and MD/XB text."""
+def reg2aln_regions(run=""):
+    """Each mem_reg2aln call of a reference run ("" = the cfg0 reads, "edge_" =
+    the edited reads) as a region (read, qb, qe, rb, re, truesc, w) -- qb/qe
+    from the read's final region with that window -- with the reference's try:
+    w2, global score, CIGAR, NM and MD/XB text."""
     z = fixture()
-    R = z["reg2aln"]
-    fin = z["final_region"]
-    tails = bytes(z["reg2aln_tail"]).decode().split("\n")
+    R = z[run + "reg2aln"]
+    fin = z[run + "final_region"]
+    tails = bytes(z[run + "reg2aln_tail"]).decode().split("\n")
     cig_off = np.concatenate([[0], np.cumsum(R[:, 12])])
     out = []
     for k, row in enumerate(R):
@@ -68,7 +69,7 @@ def reg2aln_regions():
         qb, qe = int(m[0, 2]), int(m[0, 3])
         md, _, xb = tails[k].partition("\tXB:Z:")
         out.append(dict(region=(i, qb, qe, rb, re, lsc, alnreg), w2=w2, score=gsc, n_cigar=nc, nm=nm,
-                        cigar=z["reg2aln_cigar"][cig_off[k]:cig_off[k] + nc], md=md, xb=xb,
+                        cigar=z[run + "reg2aln_cigar"][cig_off[k]:cig_off[k] + nc], md=md, xb=xb,
                         meth=[int(x) for x in row[14:18]], inferred=inf))
     return out
 
@@ -119,3 +120,41 @@ def write_fasta_fastq(d, wl, n_reads=None):
             r = acgtn[wl.oread(i)].tobytes().decode()
             f.write(f"@r{i}\n{r}\n+\n{'I' * len(r)}\n")
     return fa, fq
+
+
+def sam_records(run=""):
+    """A reference run's SAM records: list of (read, flag, pos, CIGAR string)."""
+    z = fixture()
+    cig = bytes(z[run + "sam_cigar"]).decode().split("\n")
+    return [(int(r), int(f), int(p), c) for (r, f, p), c in zip(z[run + "sam"], cig)]
+
+
+def read_lengths(run=""):
+    z = fixture()
+    return np.full(int(z["n_reads"]), 100, np.int32) if run == "" else z["edge_len"]
+
+
+def reg2aln_vs_sam(run=""):
+    """The mem_reg2aln calls of a run paired with the SAM records they became:
+    for every read with as many calls as records (XA hits add calls without
+    records), in order.  Returns (cases, records)."""
+    cases = reg2aln_regions(run)
+    recs = sam_records(run)
+    by_c, by_r = {}, {}
+    for c in cases:
+        by_c.setdefault(c["region"][0], []).append(c)
+    for r in recs:
+        if not r[1] & 4:
+            by_r.setdefault(r[0], []).append(r)
+    out_c, out_r = [], []
+    for i, cs in by_c.items():
+        rs = by_r.get(i, [])
+        if len(cs) == len(rs):
+            out_c += cs
+            out_r += rs
+    return out_c, out_r
+
+
+def cigar_str(words, clip="S"):
+    """BAM CIGAR words as SAM text, clip ops (3) printed as `clip`."""
+    return "".join(f"{w >> 4}{'MID' [w & 0xf] if (w & 0xf) < 3 else clip}" for w in words)

@@ -16,8 +16,11 @@ Steps (all in a temporary directory; nothing of the reference is kept):
     stored as arrays.
 
 The fixture holds data only: every ksw_extend2 call of every read (inputs and
-the six outputs), and for the first N_TRACE reads their chains, regions, CIGAR
-tries (bwa_gen_cigar2 inputs and outputs, including the NM, MD:Z and XB:Z text)
+the six outputs); for the first N_TRACE reads their chains and regions; for
+every read its final regions, its CIGAR tries (bwa_gen_cigar2 inputs and
+outputs, including the NM, MD:Z and XB:Z text) and its SAM records (flag, POS,
+CIGAR); the same for a second run on N_EDGE edited copies of the reads (random
+ends, deletions near an end), whose regions mem_reg2aln clips and squeezes;
 and SHA-256 digests of the index's .pac / .opac.
 
 Run:  python tests/golden/make_golden_cfg0.py   (needs oracle/_ref/bwa: `make -C oracle bwa`)
@@ -37,7 +40,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 BWA = os.path.join(ROOT, "oracle", "_ref", "bwa")
 OUT = os.path.join(HERE, "cfg0_bwamem.npz")
-N_TRACE = 4000   # reads whose chains / regions / CIGAR tries are kept
+N_TRACE = 4000   # reads whose chains / regions are kept
+N_EDGE = 2000    # edited reads of the second run (clipping and deletion squeeze)
 # BSBolt Align's bwa mem argv at its defaults (Launcher.py:77-108; Parser.py:31-115), one thread
 BSB_ARGV = ["-Y", "-A", "1", "-B", "4", "-D", "0.5", "-E", "1,1", "-L", "30,30", "-T", "10", "-U", "17",
             "-W", "0", "-c", "500", "-d", "100", "-k", "19", "-m", "50", "-r", "1.5", "-t", "1", "-w", "100",
@@ -118,12 +122,17 @@ def parse_trace(path, n_reads):
     regs = [[] for _ in range(n_reads)]
     final = [[] for _ in range(n_reads)]
     reg2aln = [[] for _ in range(n_reads)]   # per mem_reg2aln call: (inferred, cmd, alnreg, [tries])
+    sam = []                                 # (read, flag, rname, pos, cigar) of every SAM record
     cur, fin = -1, -1
     with open(path, errors="replace") as f:
         for line in f:
             line = line.rstrip("\n")
-            if line.startswith("@") or (line and line[0] == "r" and "\t" in line):
-                continue  # SAM header / records
+            if line.startswith("@"):
+                continue  # SAM header
+            if line and line[0] == "r" and "\t" in line:  # SAM record
+                f_ = line.split("\t")
+                sam.append((int(f_[0][1:]), int(f_[1]), f_[2], int(f_[3]), f_[5]))
+                continue
             m = RE_READ.match(line)
             if m:
                 cur, fin = int(m.group(1)), -1
@@ -162,7 +171,55 @@ def parse_trace(path, n_reads):
                 m = RE_TRY.match(line)
                 if m:
                     reg2aln[fin][-1][1].append(tuple(int(x) for x in m.groups()))
-    return chains, ext, regs, final, reg2aln
+    return chains, ext, regs, final, reg2aln, sam
+
+
+def edge_reads(wl, n):
+    """n reads as sequenced, edited so that mem_reg2aln clips and squeezes:
+    random 5' bases, random 3' bases, both, or a 1-3 base deletion 2-6 bases
+    from one end (seeded; built from the first n cfg0 reads)."""
+    rng = np.random.default_rng(20240)
+    out = []
+    for k in range(n):
+        r = wl.oread(k).copy()
+        kind = k % 4
+        if kind in (0, 2):
+            m = int(rng.integers(12, 36)); r[:m] = rng.integers(0, 4, m)
+        if kind in (1, 2):
+            m = int(rng.integers(12, 36)); r[len(r) - m:] = rng.integers(0, 4, m)
+        if kind == 3:
+            d, p = int(rng.integers(1, 4)), int(rng.integers(2, 7))
+            p = p if rng.random() < 0.5 else len(r) - p - d
+            r = np.concatenate([r[:p], r[p + d:]])
+        out.append(r.astype(np.uint8))
+    return out
+
+
+def tries_and_sam(Cg, reg2aln, final, sam, n_reads):
+    """mem_reg2aln's tries matched to the recorder's bwa_gen_cigar2 calls, the
+    final regions and the SAM records, as fixture arrays."""
+    want = [c for c in Cg if c["want"]]
+    calls = [(i, blk) for i in range(n_reads) for blk in reg2aln[i]]
+    assert sum(len(blk[1]) for _, blk in calls) == len(want), (sum(len(b[1]) for _, b in calls), len(want))
+    rows, cg_words, tails, k = [], [], [], 0
+    for ci, (i, (band, tries)) in enumerate(calls):
+        for ti, (w2, gsc, lsc) in enumerate(tries):
+            c = want[k]; k += 1
+            assert c["w_"] == w2 and c["score"] == gsc, (i, ti)
+            rows.append((i, ci, ti, band[0], band[1], band[2], w2, gsc, lsc, c["l_query"], c["rb"], c["re"],
+                         c["n_cigar"], c["nm"], *c["meth"]))
+            cg_words.append(c["cigar"])
+            tails.append(c["tail"])
+    F = [(i,) + r for i in range(n_reads) for r in final[i]]
+    return {
+        "final_region": np.array(F, np.int64).reshape(-1, 6),  # read, score, qb, qe, rb, re
+        "reg2aln": np.array(rows, np.int64).reshape(-1, 18),
+        "reg2aln_cigar": np.concatenate(cg_words).astype(np.uint32) if cg_words else np.zeros(0, np.uint32),
+        "reg2aln_tail": np.frombuffer("\n".join(tails).encode(), np.uint8).copy(),
+        # SAM records (mem_reg2aln's squeeze and clipping, bwamem.c:1237-1263, end up here)
+        "sam": np.array([(r, fl, p) for r, fl, _, p, _ in sam], np.int64).reshape(-1, 3),  # read, flag, pos
+        "sam_cigar": np.frombuffer("\n".join(c for *_, c in sam).encode(), np.uint8).copy(),
+    }
 
 
 def main():
@@ -189,7 +246,18 @@ def main():
             subprocess.run([BWA, "mem", "-v", "4", *BSB_ARGV, fa, os.path.join(d, "reads.fq")], check=True,
                            stdout=out, stderr=subprocess.DEVNULL, env=env)
         E, G, Cg = parse_dump(os.path.join(d, "dump.bin"))
-        chains, ext, regs, final, reg2aln = parse_trace(os.path.join(d, "trace.txt"), n_reads)
+        chains, ext, regs, final, reg2aln, sam = parse_trace(os.path.join(d, "trace.txt"), n_reads)
+        # second run: edited reads whose regions get clipped / squeezed by mem_reg2aln
+        er = edge_reads(wl, N_EDGE)
+        with open(os.path.join(d, "edge.fq"), "w") as f:
+            for i, r in enumerate(er):
+                f.write(f"@r{i}\n{ACGTN[r].tobytes().decode()}\n+\n{'I' * len(r)}\n")
+        env["BSX_DUMP"] = os.path.join(d, "edge.bin")
+        with open(os.path.join(d, "edge.txt"), "w") as out:
+            subprocess.run([BWA, "mem", "-v", "4", *BSB_ARGV, fa, os.path.join(d, "edge.fq")], check=True,
+                           stdout=out, stderr=subprocess.DEVNULL, env=env)
+        _, _, eCg = parse_dump(os.path.join(d, "edge.bin"))
+        _, _, _, efinal, ereg2aln, esam = parse_trace(os.path.join(d, "edge.txt"), N_EDGE)
     # ---- extension calls, split per read by the trace's extension lines ----
     per_read = [len(x) for x in ext]
     assert sum(per_read) == len(E), (sum(per_read), len(E))
@@ -227,26 +295,11 @@ def main():
               seed_qbeg=np.array(sq, np.int32), seed_rbeg=np.array(sb, np.int64))
     R = [(i,) + r for i in range(nt) for r in regs[i]]
     fx["region"] = np.array(R, np.int64).reshape(-1, 8)        # read, qb, qe, rb, re, score, lw, rw
-    F = [(i,) + r for i in range(nt) for r in final[i]]
-    fx["final_region"] = np.array(F, np.int64).reshape(-1, 6)  # read, score, qb, qe, rb, re
-    # mem_reg2aln: the wanted bwa_gen_cigar2 calls in order, matched to the trace's tries
-    want = [c for c in Cg if c["want"]]
-    calls = [(i, blk) for i in range(n_reads) for blk in reg2aln[i]]
-    assert sum(len(blk[1]) for _, blk in calls) == len(want), (sum(len(b[1]) for _, b in calls), len(want))
-    rows, cg_words, tails, k = [], [], [], 0
-    for ci, (i, (band, tries)) in enumerate(calls):
-        for ti, (w2, gsc, lsc) in enumerate(tries):
-            c = want[k]; k += 1
-            assert c["w_"] == w2 and c["score"] == gsc, (i, ti)
-            if i < nt:
-                rows.append((i, ci, ti, band[0], band[1], band[2], w2, gsc, lsc, c["l_query"], c["rb"], c["re"],
-                             c["n_cigar"], c["nm"], *c["meth"]))
-                cg_words.append(c["cigar"])
-                tails.append(c["tail"])
-    fx["reg2aln"] = np.array(rows, np.int64).reshape(-1, 18)
-    fx["reg2aln_cigar"] = np.concatenate(cg_words).astype(np.uint32) if cg_words else np.zeros(0, np.uint32)
-    t = "\n".join(tails).encode()
-    fx["reg2aln_tail"] = np.frombuffer(t, np.uint8).copy()
+    fx.update(tries_and_sam(Cg, reg2aln, final, sam, n_reads))
+    for k, v in tries_and_sam(eCg, ereg2aln, efinal, esam, N_EDGE).items():
+        fx["edge_" + k] = v
+    fx["edge_reads"] = np.concatenate(er)
+    fx["edge_len"] = np.array([len(r) for r in er], np.int32)
     fx["n_reads"] = np.int32(n_reads)
     fx["pac_sha256"] = np.frombuffer(hashlib.sha256(pac[:nb]).digest(), np.uint8).copy()
     fx["opac_sha256"] = np.frombuffer(hashlib.sha256(opac[:nb]).digest(), np.uint8).copy()
@@ -254,7 +307,9 @@ def main():
         [wl.oread(i) for i in range(n_reads)]).tobytes()).digest(), np.uint8).copy()
     np.savez_compressed(OUT, **fx)
     print(f"wrote {OUT}: {len(E)} ksw_extend2 calls of {n_reads} reads; {len(fx['seed_read'])} seeds, "
-          f"{len(R)} regions, {len(rows)} CIGAR tries of the first {nt} reads; {len(G)} ksw_global2 calls; "
+          f"{len(R)} regions of the first {nt} reads; {len(fx['reg2aln'])} CIGAR tries, {len(sam)} SAM records; "
+          f"edited-read run: {len(fx['edge_reg2aln'])} tries, {len(fx['edge_sam'])} SAM records; "
+          f"{len(G)} ksw_global2 calls; "
           f"{os.path.getsize(OUT) / 1e6:.2f} MB")
     wl.close()
 

@@ -149,3 +149,45 @@ def test_cigar_tries_replay_reference_reg2aln():
         assert len(set(c["region"][3] >= wl.l_pac for c in cases)) == 2  # both strands of the doubled reference
     finally:
         wl.close()
+
+
+@pytest.mark.parametrize("run", ["", "edge_"])
+def test_reg2aln_cigar_equals_reference_sam(run):
+    """mem_reg2aln's tail (bwamem.c:1237-1263): position and strand from
+    bns_depos, deletion squeeze, 5'/3' clipping.  orc_reg2aln_cigar applied to
+    the reference's bwa_gen_cigar2 CIGARs gives the CIGAR, POS and strand of the
+    reference's SAM records, for the cfg0 reads and for edited reads whose
+    regions are clipped."""
+    cases, recs = K.reg2aln_vs_sam(run)
+    lens = K.read_lengths(run)
+    L, glen = 2 * 2_000_000, 2_000_000
+    assert len(cases) > (19000 if run == "" else 1900)
+    n_clip = 0
+    for c, r in zip(cases, recs):
+        i, qb, qe, rb, re = c["region"][:5]
+        k, words, pos, rev = O.reg2aln_cigar(L, rb, re, qb, qe, int(lens[i]), c["cigar"])
+        assert (K.cigar_str(words), pos % glen + 1, bool(rev)) == (r[3], r[2], bool(r[1] & 16)), (run, i)
+        n_clip += "S" in r[3]
+    assert run == "" or n_clip > 300
+
+
+def test_reg2aln_cigar_deletion_squeeze_closed_form():
+    """The squeeze branch (bwamem.c:1239-1249), which no region of either
+    reference run reaches: a leading D is dropped and moves the position; else
+    a trailing D is dropped; with both, only the leading one goes."""
+    M, I, D = 0, 1, 2
+    w = lambda n, op: n << 4 | op
+    L = 1000
+    cases = [  # cigar, rb, re, qb, qe, l_seq -> words, pos, rev
+        ([w(3, D), w(50, M)], 100, 153, 0, 50, 50, [w(50, M)], 103, 0),
+        ([w(50, M), w(2, D)], 100, 152, 0, 50, 50, [w(50, M)], 100, 0),
+        ([w(1, D), w(50, M), w(2, D)], 100, 153, 0, 50, 50, [w(50, M), w(2, D)], 101, 0),
+        ([w(4, D), w(40, M), w(1, I)], 100, 144, 5, 46, 60, [w(5, 3), w(40, M), w(1, I), w(14, 3)], 104, 0),
+        # reverse strand: position from re - 1 mirrored, clips swapped
+        ([w(40, M), w(3, D)], 1500, 1543, 5, 45, 60, [w(15, 3), w(40, M), w(5, 3)], 2 * L - 1543, 1),
+        ([w(2, D), w(40, M)], 1500, 1542, 0, 40, 40, [w(40, M)], 2 * L - 1542 + 2, 1),
+    ]
+    for cig, rb, re, qb, qe, ls, want_w, want_pos, want_rev in cases:
+        k, words, pos, rev = O.reg2aln_cigar(L, rb, re, qb, qe, ls, np.array(cig, np.uint32))
+        assert (words, pos, rev) == (want_w, want_pos, want_rev), cig
+    assert O.reg2aln_cigar(L, 10, 5, 0, 5, 5, np.array([w(5, M)], np.uint32))[0] == -1  # re <= rb

@@ -82,3 +82,37 @@ def test_gpu_cigar_tries_replay_reference_reg2aln(engine):
                 (c["nm"], c["md"], c["xb"], c["meth"]), f"region {g}: NM/MD/XB"
     finally:
         wl.close()
+
+
+@pytest.mark.parametrize("run", ["", "edge_"])
+def test_gpu_reg2aln_cigar_equals_reference_sam(engine, run):
+    """bsx_reg2aln_cigar_device on the reference's bwa_gen_cigar2 CIGARs gives
+    the CIGAR, POS and strand of its SAM records (deletion squeeze, clipping)."""
+    cases, recs = K.reg2aln_vs_sam(run)
+    lens = K.read_lengths(run)
+    n = len(cases)
+    slot = max(len(c["cigar"]) for c in cases)
+    cig = np.zeros((n, slot), np.uint32)
+    for g, c in enumerate(cases):
+        cig[g, :len(c["cigar"])] = c["cigar"]
+    reg = np.array([c["region"][:5] for c in cases], np.int64)
+    out, nout, pos, rev = engine.reg2aln_cigar(4_000_000, reg[:, 3], reg[:, 4], reg[:, 1], reg[:, 2],
+                                               lens[reg[:, 0]], [len(c["cigar"]) for c in cases], cig)
+    for g, r in enumerate(recs):
+        assert (K.cigar_str(out[g, :nout[g]]), int(pos[g]) % 2_000_000 + 1, bool(rev[g])) == \
+            (r[3], r[2], bool(r[1] & 16)), (run, g)
+    # the squeeze branch on crafted CIGARs, against the oracle
+    from oracle import oracle as O
+    w = lambda k, op: k << 4 | op
+    crafted = [([w(3, 2), w(50, 0)], 100, 153, 0, 50, 50), ([w(50, 0), w(2, 2)], 100, 152, 0, 50, 50),
+               ([w(1, 2), w(50, 0), w(2, 2)], 100, 153, 0, 50, 50), ([w(40, 0), w(3, 2)], 1500, 1543, 5, 45, 60),
+               ([w(2, 2), w(40, 0)], 1500, 1542, 0, 40, 40)]
+    cg = np.zeros((len(crafted), 3), np.uint32)
+    for g, x in enumerate(crafted):
+        cg[g, :len(x[0])] = x[0]
+    a = np.array([x[1:] for x in crafted], np.int64)
+    out, nout, pos, rev = engine.reg2aln_cigar(1000, a[:, 0], a[:, 1], a[:, 2], a[:, 3], a[:, 4],
+                                               [len(x[0]) for x in crafted], cg)
+    for g, x in enumerate(crafted):
+        k, words, p, rv = O.reg2aln_cigar(1000, *x[1:], np.array(x[0], np.uint32))
+        assert (list(out[g, :nout[g]]), int(pos[g]), int(rev[g])) == (words, p, rv), g
