@@ -280,6 +280,154 @@ def md_leg(args, eng):
                              "frac": alg / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, "algorithmic_bytes": alg}}
 
 
+EXT_ITEM = {"query": 1, "q_off": 8, "qlen": 4, "target": 1, "t_off": 8, "tlen": 4, "h0": 4, "w": 4, "end_bonus": 4}
+
+
+def chunk_ptrs(ptrs, lo, n, item):
+    """Device addresses of tasks [lo, lo+n) of a resident batch: per-task arrays
+    offset by lo, the concatenated sequences shared (offsets are absolute)."""
+    p = dict(ptrs)
+    for k, sz in item.items():
+        if k not in ("query", "target"):
+            p[k] = ptrs[k] + lo * sz
+    p["n"] = n
+    return p
+
+
+def workload_leg(args, eng, name, cigar=False):
+    """Another workload of the configs: its mem_chain2aln batch built in rounds
+    on the GPU, then timed resident in HBM in the plain, learned and
+    no-predictor frame forms (outputs must equal the plain form's).  With
+    `cigar`, the batch is also timed as hipGraph-captured chunks of
+    args.graph_batch tasks, and the workload's CIGAR pass (every try of
+    mem_reg2aln's band doubling) runs as graph-captured chunks of the same size
+    after it."""
+    from bsbolt_amd.workload import CONFIGS, Workload, build_glb_batch
+    from bsbolt_amd.tasks import RESULT_FIELDS
+    cfg = CONFIGS[name]
+    t0 = time.time()
+    wl = Workload(cfg)
+    batch, round_res, side, rnd = wl.build_ext_batch(lambda t: eng.extend(t, cells=False))
+    glb = None
+    if cigar:
+        glb = build_glb_batch(wl, lambda t: eng.global_(t))
+    wl.close()
+    n = len(batch)
+    log(f"{name}: {wl.n_reads} reads, {n} extension tasks built in {time.time() - t0:.1f}s")
+    keep, ptrs = eng.tasks_to_device(batch)
+    outs = {k: eng.empty((n,), np.int32) for k in RESULT_FIELDS}
+    cells_d = eng.empty((n,), np.int64)
+    ptrs.update({k: v.ptr for k, v in outs.items()})
+    ptrs["cells"] = cells_d.ptr
+    ptrs["n"] = n
+    eng.extend_device(ptrs)
+    eng.sync()
+    got = {k: v.numpy() for k, v in outs.items()}
+    for k in RESULT_FIELDS:
+        if not np.array_equal(got[k], round_res[k]):
+            raise SystemExit(f"{name}: resident run disagrees with the build rounds on {k}")
+    cells = cells_d.numpy()
+    total_cells = int(cells.sum())
+    steps = max(1, args.extra_steps)
+    ms = time_steps(eng, ptrs, steps)
+    leg = {"workload": f"{name}: {cfg.genome_len // 1_000_000} Mb genome, {wl.n_reads} reads x {cfg.read_len} bp"
+                       f"{' PE' if cfg.paired else ' SE'}{' RRBS' if cfg.rrbs else ''}",
+           "tasks": n, "cells_ref": total_cells, "mean_qlen": float(batch.qlen.mean()),
+           "plain": {"value": n / (ms / 1e3), "unit": "extensions/s", "ms_per_step": ms,
+                     "gcups": total_cells / (ms / 1e3) / 1e9}}
+    log(f"{name} plain: {ms:.3f} ms/step, {total_cells / (ms / 1e3) / 1e9:.1f} GCUPS")
+    if not args.no_learned:
+        a2 = argparse.Namespace(**vars(args))
+        a2.steps, a2.warmup = steps, 1
+        lb = learned_leg(a2, eng, ptrs, outs, cells_d, got, cells, n, total_cells)
+        for x in (lb, lb.get("fixed24")):
+            if x:
+                x["speedup_vs_plain"] = ms / x["ms_per_step"]
+        leg["learned_band"] = lb
+    if cigar:
+        gb = args.graph_batch
+        # extension in graph-captured chunks over the resident batch
+        graphs = []
+        for lo in range(0, n, gb):
+            p = chunk_ptrs(ptrs, lo, min(gb, n - lo), {**EXT_ITEM, **{k: 4 for k in RESULT_FIELDS}, "cells": 8})
+            graphs.append(eng.capture_extend(p))
+        for g in graphs:
+            g.launch()
+        eng.sync()
+        for k in RESULT_FIELDS:
+            if not np.array_equal(outs[k].numpy(), got[k]):
+                raise SystemExit(f"{name}: graph-batched extension disagrees on {k}")
+        eng.timer_start()
+        for _ in range(steps):
+            for g in graphs:
+                g.launch()
+        ms_g = eng.timer_stop() / steps
+        for g in graphs:
+            g.close()
+        # the CIGAR pass: every try, as graph-captured chunks
+        gt = glb[0]
+        m = len(gt)
+        slot = int((gt.qlen + gt.tlen).max())
+        gkeep = {k: eng.to_device(np.ascontiguousarray(getattr(gt, k))) for k in
+                 ("query", "q_off", "qlen", "target", "t_off", "tlen", "w")}
+        gres = {"score": eng.empty((m,), np.int32), "n_cigar": eng.empty((m,), np.int32),
+                "cigar": eng.empty((m, slot), np.uint32)}
+        gp = {k: v.ptr for k, v in {**gkeep, **gres}.items()}
+        gp.update(n=m, cigar_slot=slot)
+        ggraphs = []
+        for lo in range(0, m, gb):
+            p = chunk_ptrs(gp, lo, min(gb, m - lo), {"q_off": 8, "qlen": 4, "t_off": 8, "tlen": 4, "w": 4,
+                                                      "score": 4, "n_cigar": 4, "cigar": 4 * slot})
+            ggraphs.append(eng.capture_global(p))
+        for g in ggraphs:
+            g.launch()
+        eng.sync()
+        mism = int((gres["score"].numpy() != glb[1]).sum() + (gres["n_cigar"].numpy() != glb[2]).sum())
+        eng.timer_start()
+        for _ in range(steps):
+            for g in ggraphs:
+                g.launch()
+        ms_c = eng.timer_stop() / steps
+        for g in ggraphs:
+            g.close()
+        band = np.minimum(gt.qlen.astype(np.int64), 2 * gt.w.astype(np.int64) + 1)
+        gcells = int((band * gt.tlen.astype(np.int64)).sum())
+        leg["graph_pass"] = {
+            "graph_batch": gb, "extension_graphs": len(graphs), "cigar_graphs": len(ggraphs),
+            "extension_ms": ms_g, "cigar_tasks": m, "cigar_ms": ms_c, "cigar_mismatches_vs_rounds": mism,
+            "value": (n + m) / ((ms_g + ms_c) / 1e3), "unit": "tasks/s (ksw_extend2 + ksw_global2)",
+            "ms_per_pass": ms_g + ms_c, "gcups_extension": total_cells / (ms_g / 1e3) / 1e9,
+            "gcups_cigar_banded": gcells / (ms_c / 1e3) / 1e9}
+        log(f"{name} graph pass: {len(graphs)} extension graphs {ms_g:.3f} ms + {len(ggraphs)} CIGAR graphs "
+            f"({m} tasks) {ms_c:.3f} ms, {mism} CIGAR mismatches")
+        del gkeep, gres
+    if not args.no_cpu:
+        from oracle import oracle as O
+        kind = "reference" if O.ref_available() else "port"
+        cpu_out, dt = O.extend2_batch(batch, nthreads=args.cpu_threads, use_ref=(kind == "reference"),
+                                      want_cells=False)
+        mism = int(sum(int((cpu_out[k] != got[k]).sum()) for k in RESULT_FIELDS))
+        leg["cpu_baseline"] = {"value": n / dt, "unit": "extensions/s", **cpu_fields(args), "kind": kind,
+                               "gcups": total_cells / dt / 1e9, "seconds": dt,
+                               "sample": f"the full batch ({n} tasks), dynamic 4096-task chunks per thread",
+                               "mismatches_vs_gpu": mism}
+        log(f"{name} CPU baseline: {n / dt / 1e6:.2f} M ext/s, {mism} mismatches")
+    del keep, outs, cells_d
+    return leg
+
+
+def host_buffer_leg(eng, batch, reps):
+    """The host-memory entry point (bsx_extend_batch: H2D copies, the kernels,
+    D2H copies) on the same batch: the PCIe-inclusive rate."""
+    eng.extend(batch, cells=False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.extend(batch, cells=False)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": len(batch) / dt, "unit": "extensions/s", "ms_per_call": dt * 1e3,
+            "note": "host buffers in and out (bsx_extend_batch), plain form"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,6 +443,10 @@ def main():
     ap.add_argument("--md-scale", type=float, default=0.05, help="fraction of the reads for the NM/MD/XB leg")
     ap.add_argument("--md-tile", type=int, default=16, help="copies of those regions in the timed NM/MD/XB batch")
     ap.add_argument("--band-model", default=None, help="band model (.npy); default bsbolt_amd/band_model.npy")
+    ap.add_argument("--extra", default="cfg3,cfg4",
+                    help="other workloads timed after the headline one (cfg4 with its graph-batched CIGAR pass)")
+    ap.add_argument("--extra-steps", type=int, default=5)
+    ap.add_argument("--graph-batch", type=int, default=262144, help="tasks per hipGraph-captured chunk")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -444,6 +596,11 @@ def main():
     }
     if not args.no_md:
         line["md_layer"] = md_leg(args, eng)
+    line["host_buffers"] = host_buffer_leg(eng, batch, 3)
+    log(f"host-buffer entry point: {line['host_buffers']['ms_per_call']:.1f} ms per call")
+    del keep, outs, cells_d
+    extra = [x for x in args.extra.split(",") if x and x != args.workload]
+    line["workloads"] = {x: workload_leg(args, eng, x, cigar=(x == "cfg4")) for x in extra}
     if learned:  # the north-star ratio: learned-band form over the plain form, and the no-predictor frame form
         learned["speedup_vs_plain"] = ms_step / learned["ms_per_step"]
         if learned.get("fixed24"):
