@@ -477,6 +477,8 @@ def main():
     if args.scale != 1.0:
         cfg = cfg.scaled(args.scale)
     eng = bsx.Engine(0)
+    tune_table, tune_src, tune_key = eng.get_tuning()
+    log(f"tuning table {tune_table} ({tune_src}, {tune_key})")
     t0 = time.time()
     wl = Workload(cfg)
     log(f"simulated {wl.n_reads} reads on a {cfg.genome_len / 1e6:.0f} Mb genome in {time.time() - t0:.1f}s")
@@ -593,6 +595,8 @@ def main():
         "roofline_hbm": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes": alg_bytes},
         "learned_band": learned,
+        "tuning": {"table": dict(zip(("tpb_plain", "tpb_narrow", "shape_var_65_96", "shape_var_97_128"), tune_table)),
+                   "source": tune_src, "device": tune_key},
     }
     if not args.no_md:
         line["md_layer"] = md_leg(args, eng)

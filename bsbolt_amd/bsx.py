@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(HERE, "libbsx.so")
 BSX_OK, BSX_EINVAL, BSX_EDEVICE, BSX_ENOMEM, BSX_ERANGE = 0, -1, -2, -3, -4
 BSX_MAX_QLEN = 2048
 BSX_NOALLOC_MAX_TASKS = 262144
+TUNE_FIELDS = 4
 _ERRNAME = {BSX_EINVAL: "BSX_EINVAL", BSX_EDEVICE: "BSX_EDEVICE", BSX_ENOMEM: "BSX_ENOMEM",
             BSX_ERANGE: "BSX_ERANGE"}
 
@@ -37,7 +38,8 @@ EXPORTS = ("bsx_create", "bsx_destroy", "bsx_last_error", "bsx_set_stream", "bsx
            "bsx_band_train_device", "bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits",
            "bsx_graph_capture_extend", "bsx_graph_capture_global", "bsx_graph_launch", "bsx_graph_destroy",
            "bsx_set_side_streams", "bsx_md_batch_device", "bsx_band_predict_device", "bsx_dropin_contexts",
-           "bsx_extend_batch_device_noalloc", "bsx_ksw_global2", "bsx_reg2aln_cigar_device")
+           "bsx_extend_batch_device_noalloc", "bsx_ksw_global2", "bsx_reg2aln_cigar_device",
+           "bsx_set_tuning", "bsx_get_tuning")
 TIMED_SLOTS = 16
 # lanes per task and columns per lane of the plain form's bins (bsx_internal.h kExtShape);
 # one lane per task is the lane-per-task kernel
@@ -182,6 +184,10 @@ def lib():
         L.bsx_md_batch_device.restype = C.c_int
         L.bsx_reg2aln_cigar_device.argtypes = [P, C.POINTER(_Reg2alnTasks), C.POINTER(_Reg2alnResults)]
         L.bsx_reg2aln_cigar_device.restype = C.c_int
+        L.bsx_set_tuning.argtypes = [P, P, C.c_int]
+        L.bsx_set_tuning.restype = C.c_int
+        L.bsx_get_tuning.argtypes = [P, P, C.c_int, C.c_char_p, C.c_int]
+        L.bsx_get_tuning.restype = C.c_int
         for f in ("bsx_set_kernel_timing", "bsx_kernel_times", "bsx_ext_bin_limits", "bsx_set_side_streams"):
             getattr(L, f).restype = C.c_int
         for f in ("bsx_set_band_policy", "bsx_band_stats", "bsx_band_model_set", "bsx_band_model_get",
@@ -520,6 +526,24 @@ class Engine:
         rs = _MdResults(dev["md"], dev["md_slot"], dev["md_len"], dev["xb"], dev["xb_slot"], dev["xb_len"],
                         dev["nm"], dev["meth"], dev["status"])
         self._check(self.L.bsx_md_batch_device(self.ctx, C.byref(tk), C.byref(rs)))
+
+    def get_tuning(self):
+        """(table [tpb_plain, tpb_narrow, shape_var_65_96, shape_var_97_128],
+        source "default" | "cache" | "caller", device key)."""
+        t = np.zeros(TUNE_FIELDS, np.int32)
+        key = C.create_string_buffer(64)
+        src = self.L.bsx_get_tuning(self.ctx, t.ctypes.data, TUNE_FIELDS, key, 64)
+        if src < 0:
+            self._check(src)
+        return [int(x) for x in t], ("default", "cache", "caller")[src], key.value.decode()
+
+    def set_tuning(self, table=None):
+        """Sets the tuning table (None: the defaults)."""
+        if table is None:
+            self._check(self.L.bsx_set_tuning(self.ctx, None, 0))
+        else:
+            t = np.ascontiguousarray(table, np.int32)
+            self._check(self.L.bsx_set_tuning(self.ctx, t.ctypes.data, len(t)))
 
     def reg2aln_cigar(self, l_pac, rb, re, qb, qe, l_seq, n_cigar, cigar):
         """SAM CIGAR and position of regions from their bwa_gen_cigar2 CIGARs

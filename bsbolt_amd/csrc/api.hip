@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <dlfcn.h>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -96,6 +97,11 @@ struct bsx_ctx {
   bool pending_check = false;
   int pending_kind = 0;  // 1 = extension, 2 = global
   void* scratch = nullptr;  // this device's scratch array (allocation-free entry point)
+  // tuning table (bsx_get_tuning layout): tpb_plain, tpb_narrow, shape variant of
+  // the 65-96 and 97-128 bins; where it came from; the device's key in the cache
+  int32_t tune[BSX_TUNE_FIELDS] = {kExtTasksPerBlock, kNarrowTasksPerBlock, 0, 0};
+  int tune_source = BSX_TUNE_DEFAULT;
+  char tune_key[64] = {0};
 };
 
 static int fail(bsx_ctx* c, int code, const char* fmt, ...) {
@@ -173,6 +179,58 @@ static void* scratch_address(int device) {
   return g_scratch_ptr[device];
 }
 
+static bool tune_valid(const int32_t* t) {
+  return t[0] >= 16 && t[0] <= 4096 && t[1] >= 16 && t[1] <= 4096 && (t[2] == 0 || t[2] == 1) &&
+         (t[3] == 0 || t[3] == 1);
+}
+
+// The autotuner's cache (written by `python -m bsbolt_amd.tune`): a JSON object
+// keyed by "<gfx arch>/<CU count>", each entry holding "tpb_plain",
+// "tpb_narrow" and "shape_var": [v0, v1].  The file is $BSX_TUNE, else
+// tune.json next to libbsx.so; BSX_TUNE=off keeps the defaults.  Only this
+// flat layout is read; an entry that does not parse leaves the defaults.
+static void load_tuning(bsx_ctx* c) {
+  const char* env = getenv("BSX_TUNE");
+  if (env && !strcmp(env, "off")) return;
+  std::string path;
+  if (env && *env) {
+    path = env;
+  } else {
+    Dl_info info;
+    if (!dladdr((void*)&load_tuning, &info) || !info.dli_fname) return;
+    path = info.dli_fname;
+    const size_t slash = path.rfind('/');
+    path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tune.json";
+  }
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return;
+  std::string js;
+  char buf[4096];
+  size_t k;
+  while ((k = fread(buf, 1, sizeof buf, f)) > 0) js.append(buf, k);
+  fclose(f);
+  const std::string key = std::string("\"") + c->tune_key + "\"";
+  size_t p = js.find(key);
+  if (p == std::string::npos) return;
+  const size_t e = js.find('}', p);
+  if (e == std::string::npos) return;
+  const std::string obj = js.substr(p, e - p);
+  auto num = [&](const char* name, int32_t* out) -> bool {
+    const size_t q = obj.find(std::string("\"") + name + "\"");
+    if (q == std::string::npos) return false;
+    const size_t colon = obj.find(':', q);
+    return colon != std::string::npos && sscanf(obj.c_str() + colon + 1, " %d", out) == 1;
+  };
+  int32_t t[BSX_TUNE_FIELDS];
+  const size_t q = obj.find("\"shape_var\"");
+  const size_t br = q == std::string::npos ? q : obj.find('[', q);
+  if (!num("tpb_plain", &t[0]) || !num("tpb_narrow", &t[1]) || br == std::string::npos ||
+      sscanf(obj.c_str() + br + 1, " %d , %d", &t[2], &t[3]) != 2 || !tune_valid(t))
+    return;
+  memcpy(c->tune, t, sizeof t);
+  c->tune_source = BSX_TUNE_CACHE;
+}
+
 extern "C" {
 
 const char* bsx_version(void) { return "bsx 0.1 (gfx950)"; }
@@ -220,10 +278,15 @@ bsx_ctx* bsx_create(int device) {
   c->err_host[1] = INT_MAX;
   c->err_host[2] = 0;
   c->scratch = scratch_address(device);  // once per device; nullptr leaves only the no-alloc call failing
-  // work granularity of the extension kernels (tuning knob; defaults in bsx_internal.h)
-  const char* tp = getenv("BSX_TPB_PLAIN");
-  const char* tn = getenv("BSX_TPB_NARROW");
-  set_tasks_per_block(tp ? atoi(tp) : kExtTasksPerBlock, tn ? atoi(tn) : kNarrowTasksPerBlock);
+  // tuning table: the cache entry for this device, else the defaults
+  {
+    char arch[64];
+    snprintf(arch, sizeof arch, "%s", prop.gcnArchName);
+    for (char* p = arch; *p; ++p)
+      if (*p == ':') { *p = 0; break; }
+    snprintf(c->tune_key, sizeof c->tune_key, "%s/%d", arch, c->num_cu);
+    load_tuning(c);
+  }
   return c;
 }
 
@@ -306,7 +369,33 @@ static ExtWork ext_work(bsx_ctx* c) {
   wk.nbin_of = (uint8_t*)c->ws->nbin_of.p;
   wk.need_plain = (uint8_t*)c->ws->need_plain.p;
   wk.ss = c->ss_on ? &c->ss : nullptr;
+  wk.tpb_plain = c->tune[0];
+  wk.tpb_narrow = c->tune[1];
+  wk.shape_var[0] = c->tune[2];
+  wk.shape_var[1] = c->tune[3];
   return wk;
+}
+
+int bsx_set_tuning(bsx_ctx* c, const int32_t* table, int n) {
+  if (!c) return BSX_EINVAL;
+  if (!table) {  // back to the defaults
+    const int32_t d[BSX_TUNE_FIELDS] = {kExtTasksPerBlock, kNarrowTasksPerBlock, 0, 0};
+    memcpy(c->tune, d, sizeof d);
+    c->tune_source = BSX_TUNE_DEFAULT;
+    return BSX_OK;
+  }
+  if (n != BSX_TUNE_FIELDS || !tune_valid(table))
+    return fail(c, BSX_EINVAL, "tuning table: %d fields, tpb in [16, 4096], shape variants 0 or 1", BSX_TUNE_FIELDS);
+  memcpy(c->tune, table, sizeof c->tune);
+  c->tune_source = BSX_TUNE_CALLER;
+  return BSX_OK;
+}
+
+int bsx_get_tuning(const bsx_ctx* c, int32_t* table, int n, char* key, int key_cap) {
+  if (!c || !table || n < BSX_TUNE_FIELDS) return BSX_EINVAL;
+  memcpy(table, c->tune, sizeof c->tune);
+  if (key && key_cap > 0) snprintf(key, (size_t)key_cap, "%s", c->tune_key);
+  return c->tune_source;
 }
 
 static int ext_args(bsx_ctx* c, const bsx_scoring* sc, const bsx_ext_tasks* t, ExtArgs& a) {

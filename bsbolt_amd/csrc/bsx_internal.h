@@ -159,7 +159,22 @@ struct ExtWork {
   hipEvent_t* tev = nullptr;
   uint32_t* tmask = nullptr;
   const StreamSet* ss = nullptr;  // side streams (nullable: everything on the call's stream)
+  // tuning: descriptors per block of the plain group kernels and of the frame
+  // group kernels, and the shape variant of the bins with alternatives
+  int tpb_plain = kExtTasksPerBlock, tpb_narrow = kNarrowTasksPerBlock;
+  int shape_var[2] = {0, 0};
 };
+// Bins with alternative group shapes (same columns per task, so the binning is
+// unchanged; the tuner picks one per device): index 0 = bin 4 (queries of
+// 65-96 bases), 1 = bin 5 (97-128).  Variant 0 is kExtShape's shape.
+constexpr int kTunedBin0 = 4;
+constexpr BinShape kAltShape[2][2] = {{{16, 6}, {32, 3}}, {{16, 8}, {32, 4}}};
+static_assert(kExtShape[kTunedBin0].lanes == kAltShape[0][0].lanes &&
+              kExtShape[kTunedBin0].cols == kAltShape[0][0].cols &&
+              kExtShape[kTunedBin0 + 1].lanes == kAltShape[1][0].lanes &&
+              kExtShape[kTunedBin0 + 1].cols == kAltShape[1][0].cols &&
+              kAltShape[0][1].lanes * kAltShape[0][1].cols == bin_limit(kTunedBin0) &&
+              kAltShape[1][1].lanes * kAltShape[1][1].cols == bin_limit(kTunedBin0 + 1), "alternative shapes");
 constexpr int kTimedSlots = 16;  // kExtBins + 4 narrow classes + predictor
 static_assert(kExtBins + kBandClasses <= kTimedSlots, "timing slots");
 inline void time_begin(const ExtWork& wk, int slot, hipStream_t st) {
@@ -182,9 +197,6 @@ template <int WF, bool E1>
 void launch_lane_frame(const ExtKArgs& k, const ExtWork& wk, int cls, int num_cu, hipStream_t st);
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts, const BandPlan* plan);
-// Tasks per block of the plain and narrow kernels (defaults above; a context
-// may override them, e.g. from BSX_TPB_PLAIN / BSX_TPB_NARROW).
-void set_tasks_per_block(int plain, int narrow);
 hipError_t launch_narrow(const ExtArgs& a, const ExtKArgs& k, const ExtWork& wk, int num_cu, hipStream_t st,
                          const BandPlan& plan);
 

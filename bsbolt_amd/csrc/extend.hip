@@ -458,22 +458,33 @@ ext_unpack_kernel(ExtArgs A, ExtWork W, const int32_t* __restrict__ rec) {
   }
 }
 
+// one block per k.tpb descriptors of bin B; without a host-side count the
+// grid covers the whole batch and surplus blocks exit at once
+template <int L, int C, bool E1>
+static void launch_group(const ExtKArgs& k, const ExtWork& wk, int B, int64_t n, hipStream_t st) {
+  const int64_t blocks = (n + k.tpb - 1) / k.tpb;
+  if (blocks < 1) return;
+  time_begin(wk, B, st);
+  hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
+                     (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
+  time_end(wk, B, st);
+}
+
 template <int B, bool E1>
 static void launch_bin(const ExtKArgs& k, const ExtWork& wk, int64_t n_all, int num_cu, hipStream_t st,
                        int64_t hint) {
   constexpr int L = kExtShape[B].lanes, C = kExtShape[B].cols;
+  const int64_t n = hint >= 0 ? hint : n_all;
   if constexpr (L == 1) {
     launch_lane_bin<C, E1>(k, wk, B, num_cu, st, hint);
+  } else if constexpr (B == kTunedBin0 || B == kTunedBin0 + 1) {
+    constexpr int V = B - kTunedBin0;
+    if (wk.shape_var[V] == 1)
+      launch_group<kAltShape[V][1].lanes, kAltShape[V][1].cols, E1>(k, wk, B, n, st);
+    else
+      launch_group<L, C, E1>(k, wk, B, n, st);
   } else {
-    // one block per k.tpb descriptors of the bin; without a host-side
-    // count the grid covers the whole batch and surplus blocks exit at once
-    const int64_t n = hint >= 0 ? hint : n_all;
-    const int64_t blocks = (n + k.tpb - 1) / k.tpb;
-    if (blocks < 1) return;
-    time_begin(wk, B, st);
-    hipLaunchKernelGGL((ext_plain_kernel<L, C, E1>), dim3((unsigned)blocks), dim3(kExtBlock), 0, st, k,
-                       (const ExtDesc*)wk.desc, (const uint32_t*)wk.counts, B);
-    time_end(wk, B, st);
+    launch_group<L, C, E1>(k, wk, B, n, st);
   }
 }
 
@@ -491,13 +502,6 @@ static void launch_bins(const ExtKArgs& k, const ExtWork& wk, int64_t n, int num
   launch_bins_from<kExtBins - 1, E1>(k, wk, n, num_cu, st, hc);
 }
 
-static int g_tpb_plain = kExtTasksPerBlock, g_tpb_narrow = kNarrowTasksPerBlock;
-
-void set_tasks_per_block(int plain, int narrow) {
-  if (plain >= 16 && plain <= 65536) g_tpb_plain = plain;
-  if (narrow >= 16 && narrow <= 65536) g_tpb_narrow = narrow;
-}
-
 hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStream_t st,
                          const int64_t* host_bin_counts, const BandPlan* plan) {
   hipLaunchKernelGGL(ext_reset_kernel, dim3(1), dim3(kFetchSlots), 0, st, wk);
@@ -511,7 +515,7 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   ExtKArgs k{};
   k.query = a.query; k.target = a.target; k.rec = wk.rec;
   k.o_del = a.o_del; k.e_del = a.e_del; k.o_ins = a.o_ins; k.e_ins = a.e_ins; k.zdrop = a.zdrop;
-  k.tpb = g_tpb_plain;
+  k.tpb = wk.tpb_plain;
   for (int q = 0; q < 5; ++q) { k.prof_lo[q] = a.prof_lo[q]; k.prof_hi[q] = a.prof_hi[q]; }
   for (int t = 0; t < 5; ++t) {  // the same matrix by target code
     uint32_t lo = 0;
@@ -525,7 +529,7 @@ hipError_t launch_extend(const ExtArgs& a, const ExtWork& wk, int num_cu, hipStr
   hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)sblocks), dim3(256), 0, st, a, wk, band ? 1 : 0);
   if (band) {
     ExtKArgs kn = k;
-    kn.tpb = g_tpb_narrow;
+    kn.tpb = wk.tpb_narrow;
     hipError_t e = launch_narrow(a, kn, wk, num_cu, st, *plan);
     if (e != hipSuccess || plan->skip_plain) return e;
     hipLaunchKernelGGL(ext_bin_count_kernel, dim3((unsigned)sblocks), dim3(256), 0, st, a, wk, 2);

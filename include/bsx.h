@@ -330,6 +330,23 @@ int bsx_ext_bin_limits(int32_t *limits, int n);
  * roofline).  No counterpart in the reference. */
 int bsx_set_side_streams(bsx_ctx *ctx, int enable);
 
+/* ---- tuning table (no counterpart in the reference) ----
+ * table[0] descriptors per block of the plain group kernels, table[1] of the
+ * frame group kernels (both 16..4096), table[2] / table[3] the group shape of
+ * the bins of queries 65-96 / 97-128 bases (0: 16 lanes x 6 / 8 columns,
+ * 1: 32 lanes x 3 / 4 columns).  Every table gives the same results.
+ * bsx_create loads the autotuner's cache entry for the device (key
+ * "<gfx arch>/<CU count>" in $BSX_TUNE, else tune.json next to libbsx.so;
+ * BSX_TUNE=off keeps the defaults).  bsx_set_tuning(ctx, NULL, 0) restores the
+ * defaults.  bsx_get_tuning writes the table and the device's key and returns
+ * where the table came from (BSX_TUNE_*), or BSX_EINVAL. */
+#define BSX_TUNE_FIELDS  4
+#define BSX_TUNE_DEFAULT 0
+#define BSX_TUNE_CACHE   1
+#define BSX_TUNE_CALLER  2
+int bsx_set_tuning(bsx_ctx *ctx, const int32_t *table, int n);
+int bsx_get_tuning(const bsx_ctx *ctx, int32_t *table, int n, char *key, int key_cap);
+
 /* Scalar drop-in with the exact signature of ksw_extend2 (ksw.h:108), run as
  * a one-task batch on device 0.  m must be 5.  It returns INT32_MIN and leaves
  * *qle, *tle, *gtle, *gscore and *max_off untouched in these cases, which the

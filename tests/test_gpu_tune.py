@@ -1,0 +1,63 @@
+"""The tuning table: every value of every knob gives the oracle's results,
+and the autotuner's cached table for this device (bsbolt_amd/tune.json) is
+bit-exact and not slower than the default table on the 150 bp workload."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("score", "qle", "tle", "gtle", "gscore", "max_off", "cells")
+
+
+@pytest.mark.parametrize("table", [[16, 16, 1, 0], [64, 32, 0, 1], [128, 64, 1, 1]])
+def test_every_table_is_bit_exact(engine, table):
+    from bsbolt_amd import bsx
+    from bsbolt_amd.tasks import random_ext_tasks
+    t = random_ext_tasks(30000, seed=91, qmax=140, h0max=120, w_choices=(16, 100), n_frac=0.001)
+    want, _ = O.extend2_batch(t, nthreads=16)
+    old = engine.get_tuning()[0]
+    try:
+        engine.set_tuning(table)
+        for pol in ((bsx.BAND_PLAIN, 0), (bsx.BAND_FIXED, 2)):
+            engine.set_band_policy(*pol)
+            got = engine.extend(t)
+            for k in FIELDS:
+                assert np.array_equal(got[k], want[k]), f"table {table}, policy {pol}: {k}"
+    finally:
+        engine.set_band_policy(bsx.BAND_PLAIN)
+        engine.set_tuning(old)
+
+
+def test_cached_table_is_exact_and_not_slower(engine):
+    from bsbolt_amd import tune
+    from bsbolt_amd.tasks import RESULT_FIELDS
+    from bsbolt_amd.workload import CONFIGS, Workload
+    import json
+    table, src, key = engine.get_tuning()
+    cache = json.load(open(tune.CACHE)) if os.path.exists(tune.CACHE) else {}
+    if key not in cache:
+        pytest.skip(f"no cached tuning for {key}")
+    assert src == "cache" and table == [cache[key]["tpb_plain"], cache[key]["tpb_narrow"], *cache[key]["shape_var"]]
+    wl = Workload(CONFIGS["cfg1"].scaled(0.25))
+    batch = wl.build_ext_batch(lambda t: engine.extend(t, cells=False))[0]
+    wl.close()
+    n = len(batch)
+    keep, ptrs = engine.tasks_to_device(batch)
+    outs = {k: engine.empty((n,), np.int32) for k in RESULT_FIELDS}
+    ptrs.update({k: v.ptr for k, v in outs.items()})
+    ptrs["n"] = n
+    res, ms = {}, {}
+    try:
+        for name, tb in (("default", None), ("tuned", table)):
+            engine.set_tuning(tb)
+            ms[name] = tune._time(engine, ptrs, 5, reps=5)
+            res[name] = {k: v.numpy() for k, v in outs.items()}
+    finally:
+        engine.set_tuning(table)
+    for k in RESULT_FIELDS:
+        assert np.array_equal(res["default"][k], res["tuned"][k]), k
+    print(f"cfg1 x 0.25 ({n} tasks): default {ms['default']:.3f} ms, tuned {table} {ms['tuned']:.3f} ms", flush=True)
+    assert ms["tuned"] <= ms["default"] * 1.03
