@@ -185,10 +185,11 @@ static bool tune_valid(const int32_t* t) {
 }
 
 // The autotuner's cache (written by `python -m bsbolt_amd.tune`): a JSON object
-// keyed by "<gfx arch>/<CU count>", each entry holding "tpb_plain",
-// "tpb_narrow" and "shape_var": [v0, v1].  The file is $BSX_TUNE, else
-// tune.json next to libbsx.so; BSX_TUNE=off keeps the defaults.  Only this
-// flat layout is read; an entry that does not parse leaves the defaults.
+// keyed by "<gfx arch>/<CU count>", each entry an object holding
+// "bsx_table": [tpb_plain, tpb_narrow, shape variant 65-96, shape variant
+// 97-128] beside the tuner's timings.  The file is $BSX_TUNE, else tune.json
+// next to libbsx.so; BSX_TUNE=off keeps the defaults.  An entry that does not
+// parse, or a table outside the valid ranges, leaves the defaults.
 static void load_tuning(bsx_ctx* c) {
   const char* env = getenv("BSX_TUNE");
   if (env && !strcmp(env, "off")) return;
@@ -210,22 +211,16 @@ static void load_tuning(bsx_ctx* c) {
   while ((k = fread(buf, 1, sizeof buf, f)) > 0) js.append(buf, k);
   fclose(f);
   const std::string key = std::string("\"") + c->tune_key + "\"";
-  size_t p = js.find(key);
+  const size_t p = js.find(key);
   if (p == std::string::npos) return;
-  const size_t e = js.find('}', p);
-  if (e == std::string::npos) return;
-  const std::string obj = js.substr(p, e - p);
-  auto num = [&](const char* name, int32_t* out) -> bool {
-    const size_t q = obj.find(std::string("\"") + name + "\"");
-    if (q == std::string::npos) return false;
-    const size_t colon = obj.find(':', q);
-    return colon != std::string::npos && sscanf(obj.c_str() + colon + 1, " %d", out) == 1;
-  };
+  // the entry's table: the first "bsx_table": [n, n, n, n] after the device key
+  // (the tuner writes each device's entry as one object holding it)
+  const size_t q = js.find("\"bsx_table\"", p);
+  const size_t br = q == std::string::npos ? q : js.find('[', q);
   int32_t t[BSX_TUNE_FIELDS];
-  const size_t q = obj.find("\"shape_var\"");
-  const size_t br = q == std::string::npos ? q : obj.find('[', q);
-  if (!num("tpb_plain", &t[0]) || !num("tpb_narrow", &t[1]) || br == std::string::npos ||
-      sscanf(obj.c_str() + br + 1, " %d , %d", &t[2], &t[3]) != 2 || !tune_valid(t))
+  if (br == std::string::npos ||
+      sscanf(js.c_str() + br + 1, " %d , %d , %d , %d", &t[0], &t[1], &t[2], &t[3]) != BSX_TUNE_FIELDS ||
+      !tune_valid(t))
     return;
   memcpy(c->tune, t, sizeof t);
   c->tune_source = BSX_TUNE_CACHE;

@@ -100,7 +100,7 @@ def main():
     print(f"tuning {key} on {args.workload} x {args.scale}: {len(batch)} tasks", flush=True)
     table, rec = tune(eng, batch, args.steps, log=lambda m: print(m, flush=True))
     cache = json.load(open(args.out)) if os.path.exists(args.out) else {}
-    cache[key] = {"tpb_plain": table[0], "tpb_narrow": table[1], "shape_var": [table[2], table[3]],
+    cache[key] = {"bsx_table": table,
                   "workload": f"{args.workload} x {args.scale} ({len(batch)} tasks)", **rec}
     with open(args.out, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)
