@@ -232,3 +232,25 @@ def test_adam_first_step_is_sign_step():
     p1, m1, v1 = B.adam(p, m, v, g, 1, 0.01)
     assert np.allclose(p1, p - 0.01 * np.sign(g), atol=1e-6)
     assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
+
+
+def test_adam_constant_gradient_many_steps():
+    """With a constant gradient the bias-corrected moments are exactly g and g^2
+    at every step t, so each step moves by lr * g / (|g| + eps): after T steps
+    p = p0 - T * lr * g / (|g| + eps).  A bias correction with the wrong step
+    count (e.g. t stuck at 1, or t off by one) breaks this from step 2 on."""
+    rng = np.random.default_rng(21)
+    p0 = rng.normal(0, 1, 300).astype(np.float32)
+    g = rng.normal(0, 1, 300).astype(np.float32)
+    g[np.abs(g) < 1e-3] = 1e-3
+    p, m, v = p0.copy(), np.zeros_like(p0), np.zeros_like(p0)
+    lr, T = 1e-3, 25
+    for t in range(1, T + 1):
+        p, m, v = B.adam(p, m, v, g, t, lr)
+    want = p0 - T * lr * g / (np.abs(g) + 1e-8)
+    assert np.abs(p - want).max() < 2e-5
+    # a bias correction stuck at step 1 drifts away from the closed form
+    q, m2, v2 = p0.copy(), np.zeros_like(p0), np.zeros_like(p0)
+    for t in range(1, T + 1):
+        q, m2, v2 = B.adam(q, m2, v2, g, 1, lr)
+    assert np.abs(q - want).max() > 1e-3
