@@ -81,7 +81,9 @@ def test_global_cigar_slot_overflow(engine):
 
 def test_global_long_queries_narrow_and_wide_bands(engine):
     """Queries longer than any group (the lanes hold the band, not the query),
-    w = 0, and bands wider than both sequences (taken as the full matrix)."""
+    w = 0, and bands wider than both sequences (taken as the full matrix).  The session engine keeps its side streams on, so the tasks with
+    bands of 25-64 columns also run on the side stream's own slot region with
+    this enlarged slot size (its offset scales with the slot)."""
     from bsbolt_amd.tasks import GlbTasks
     rng = np.random.default_rng(45)
     pairs, ws = [], []
