@@ -165,6 +165,61 @@ wl_ctx* wl_create(const int32_t* opt10, const uint8_t* pac, int64_t glen) {
 
 void wl_destroy(wl_ctx* c) { delete c; }
 
+// ---------------- undirectional libraries (-z) ----------------
+
+// Conversion type of a read (pair) of an undirectional library
+// (bs_helpers.cpp:41-62): 0 = C->T, 1 = G->A, 2 = align both patterns.  The
+// C and G shares of the read (for a pair, read 2 counted with C and G swapped,
+// as an Illumina pair's second read is the other strand) are compared in
+// single precision, as the reference does.
+int wl_assess_conversion(const char* s1, const char* s2, int paired, float sub) {
+  float observed = (float)strlen(s1);
+  float c = 0, g = 0;
+  for (const char* p = s1; *p; ++p) {
+    c += *p == 'C';
+    g += *p == 'G';
+  }
+  if (paired) {
+    observed += (float)strlen(s2);
+    for (const char* p = s2; *p; ++p) {
+      g += *p == 'C';
+      c += *p == 'G';
+    }
+  }
+  const float cp = c / observed, gp = g / observed;
+  float d = cp - gp;
+  if (d < 0) d = d * -1.0f;
+  if (cp > sub && gp > sub) return 2;
+  if (cp == gp) return 2;
+  if (d < 0.02) return 2;
+  return cp < gp ? 0 : 1;
+}
+
+// The read records bseq_read (bwa.c:73-133) makes for one read, or one pair
+// (s2 non-NULL), as (first, read group, substitution pattern) triples in
+// order; returns their count (1, 2 or 4).  Directional libraries: read 1 C->T,
+// read 2 G->A.  Undirectional: the pattern wl_assess_conversion picks, read 2
+// the other one; when it picks both, the pattern-0 records come first, then a
+// second read group with the patterns swapped.
+int wl_bseq_records(const char* s1, const char* s2, int undirectional, float sub, int32_t* out) {
+  int pattern = 0, both = 0, n = 0;
+  if (undirectional) {
+    const int t = s2 ? wl_assess_conversion(s1, s2, 1, sub) : wl_assess_conversion(s1, s1, 0, sub);
+    if (t == 2) both = 1;
+    else pattern = t;
+  }
+  auto put = [&](int first, int group, int pat) {
+    out[3 * n] = first; out[3 * n + 1] = group; out[3 * n + 2] = pat; ++n;
+  };
+  put(0, 0, pattern);
+  if (s2) put(1, 0, pattern ? 0 : 1);
+  if (both) {
+    put(0, 1, 1);
+    if (s2) put(1, 1, 0);
+  }
+  return n;
+}
+
 // ---------------- reads ----------------
 
 // Simulates n fragments.  cfg (doubles): [0] read_len, [1] paired (0/1),

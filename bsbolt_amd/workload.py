@@ -108,6 +108,10 @@ def wl():
         L.wl_n_reads.argtypes = [_P]
         L.wl_get_seq.restype = C.c_int64
         L.wl_get_seq.argtypes = [_P, C.c_int64, C.c_int64, _P]
+        L.wl_assess_conversion.restype = C.c_int
+        L.wl_assess_conversion.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_float]
+        L.wl_bseq_records.restype = C.c_int
+        L.wl_bseq_records.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_float, _P]
         _wl = L
     return _wl
 
@@ -427,6 +431,24 @@ def md_inputs(wl, regions, owner, cigars):
         cig[t, :len(c)] = c
     return (idx, rb, re, np.concatenate(parts).astype(np.uint8) if parts else np.zeros(1, np.uint8), oq_off,
             l_query, np.array([len(c) for c in cg], np.int32), cig)
+
+
+def assess_conversion(seq1, seq2=None, substitution_proportion=0.1):
+    """Conversion type of an undirectional read or pair (bs_helpers.cpp:41-62):
+    0 = C->T, 1 = G->A, 2 = align both patterns.  Sequences as ASCII strings."""
+    s1 = seq1.encode()
+    s2 = seq2.encode() if seq2 is not None else s1
+    return wl().wl_assess_conversion(s1, s2, 1 if seq2 is not None else 0, substitution_proportion)
+
+
+def bseq_records(seq1, seq2=None, undirectional=False, substitution_proportion=0.1):
+    """The read records bwa mem makes for a read or pair (bwa.c:73-133), as
+    (first, read group, substitution pattern) triples: pattern 0 converts
+    C->T, 1 converts G->A (bs_helpers.cpp:18-31)."""
+    out = np.zeros(12, np.int32)
+    n = wl().wl_bseq_records(seq1.encode(), seq2.encode() if seq2 is not None else None, int(undirectional),
+                             substitution_proportion, _p(out))
+    return [tuple(int(x) for x in out[3 * k:3 * k + 3]) for k in range(n)]
 
 
 def pack2(codes):

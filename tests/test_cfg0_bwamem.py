@@ -191,3 +191,38 @@ def test_reg2aln_cigar_deletion_squeeze_closed_form():
         k, words, pos, rev = O.reg2aln_cigar(L, rb, re, qb, qe, ls, np.array(cig, np.uint32))
         assert (words, pos, rev) == (want_w, want_pos, want_rev), cig
     assert O.reg2aln_cigar(L, 10, 5, 0, 5, 5, np.array([w(5, M)], np.uint32))[0] == -1  # re <= rb
+
+
+def test_undirectional_conversion_equals_reference_assessconversion():
+    """The -z library mode: the producer's conversion-type choice equals the
+    compiled assessConversion (bs_helpers.cpp:41-62) on reads and pairs of every
+    composition, including the ties and the 0.02 margin; the records bseq_read
+    makes (bwa.c:73-133) follow from it."""
+    import ctypes
+    import os
+    from oracle.oracle import BWAREF_PATH
+    from bsbolt_amd.workload import assess_conversion, bseq_records
+    rng = np.random.default_rng(77)
+    cases = []
+    for _ in range(3000):
+        n1 = int(rng.integers(1, 160))
+        p = rng.dirichlet(np.ones(4) * float(rng.choice([0.3, 1.0, 5.0])))
+        s1 = "".join(rng.choice(list("ACGTN"), n1, p=list(p * 0.98) + [0.02]))
+        s2 = "".join(rng.choice(list("ACGT"), int(rng.integers(1, 160)), p=p)) if rng.random() < 0.5 else None
+        cases.append((s1, s2, float(rng.choice([0.05, 0.1, 0.3]))))
+    cases += [("ACGT", None, 0.1), ("CCGG", None, 0.6), ("CT", "CT", 0.6), ("TTTT", None, 0.1), ("CCCC", None, 0.5)]
+    ours = [assess_conversion(a, b, sp) for a, b, sp in cases]
+    assert set(ours) == {0, 1, 2}
+    if os.path.exists(BWAREF_PATH):
+        R = ctypes.CDLL(BWAREF_PATH)
+        R.assessConversion.restype = ctypes.c_int
+        R.assessConversion.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_float]
+        for (a, b, sp), got in zip(cases, ours):
+            want = R.assessConversion(a.encode(), (b or a).encode(), int(b is not None), sp)
+            assert got == want, (a, b, sp)
+    # records: directional read 1 C->T, read 2 G->A; undirectional per the type
+    assert bseq_records("ACGT", "ACGT") == [(0, 0, 0), (1, 0, 1)]
+    assert bseq_records("TTTTTG", None, True) == [(0, 0, 0)]  # G but no C left: a C->T read
+    assert bseq_records("AAAAC", None, True) == [(0, 0, 1)]   # C but no G left: a G->A read
+    assert bseq_records("CCCCA", "GGGGA", True) == [(0, 0, 1), (1, 0, 0)]
+    assert bseq_records("ACGT", "ACGT", True) == [(0, 0, 0), (1, 0, 1), (0, 1, 1), (1, 1, 0)]
