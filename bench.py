@@ -346,24 +346,41 @@ def workload_leg(args, eng, name, cigar=False):
         leg["learned_band"] = lb
     if cigar:
         gb = args.graph_batch
-        # extension in graph-captured chunks over the resident batch
-        graphs = []
-        for lo in range(0, n, gb):
-            p = chunk_ptrs(ptrs, lo, min(gb, n - lo), {**EXT_ITEM, **{k: 4 for k in RESULT_FIELDS}, "cells": 8})
-            graphs.append(eng.capture_extend(p))
-        for g in graphs:
-            g.launch()
-        eng.sync()
-        for k in RESULT_FIELDS:
-            if not np.array_equal(outs[k].numpy(), got[k]):
-                raise SystemExit(f"{name}: graph-batched extension disagrees on {k}")
-        eng.timer_start()
-        for _ in range(steps):
-            for g in graphs:
+        # extension in graph-captured chunks over the resident batch: one graph
+        # per chunk, replayed on one stream, then spread round-robin over
+        # args.graph_streams contexts (one stream each) so that a chunk's tail
+        # overlaps the next chunk's work
+        from bsbolt_amd import bsx as _bsx
+        ctxs = [eng] + [_bsx.Engine(0) for _ in range(max(0, args.graph_streams - 1))]
+        chunks = [chunk_ptrs(ptrs, lo, min(gb, n - lo), {**EXT_ITEM, **{k: 4 for k in RESULT_FIELDS}, "cells": 8})
+                  for lo in range(0, n, gb)]
+        timing = {}
+        for mode, pool in (("one_stream", [eng]), ("streams", ctxs)):
+            for e in pool:  # with --graph-in-order the chunks' graphs hold no side-stream fork/join
+                e.set_side_streams(not args.graph_in_order)
+            graphs = [(pool[j % len(pool)], pool[j % len(pool)].capture_extend(p)) for j, p in enumerate(chunks)]
+            for e in pool:
+                e.set_side_streams(True)
+            for e, g in graphs:
                 g.launch()
-        ms_g = eng.timer_stop() / steps
-        for g in graphs:
-            g.close()
+            for e in pool:
+                e.sync()
+            for k in RESULT_FIELDS:
+                if not np.array_equal(outs[k].numpy(), got[k]):
+                    raise SystemExit(f"{name}: graph-batched extension ({mode}) disagrees on {k}")
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                for e, g in graphs:
+                    g.launch()
+            for e in pool:
+                e.sync()
+            timing[mode] = (time.perf_counter() - t1) * 1e3 / steps
+            for e, g in graphs:
+                g.close()
+        for e in ctxs[1:]:
+            e.close()
+        ms_g = timing["streams"]
+        graphs = chunks
         # the CIGAR pass: every try, as graph-captured chunks
         gt = glb[0]
         m = len(gt)
@@ -394,11 +411,14 @@ def workload_leg(args, eng, name, cigar=False):
         gcells = int((band * gt.tlen.astype(np.int64)).sum())
         leg["graph_pass"] = {
             "graph_batch": gb, "extension_graphs": len(graphs), "cigar_graphs": len(ggraphs),
-            "extension_ms": ms_g, "cigar_tasks": m, "cigar_ms": ms_c, "cigar_mismatches_vs_rounds": mism,
+            "graph_streams": args.graph_streams, "graph_in_order": bool(args.graph_in_order), "extension_ms": ms_g,
+            "extension_ms_one_stream": timing["one_stream"], "extension_ms_one_call": ms,
+            "cigar_tasks": m, "cigar_ms": ms_c, "cigar_mismatches_vs_rounds": mism,
             "value": (n + m) / ((ms_g + ms_c) / 1e3), "unit": "tasks/s (ksw_extend2 + ksw_global2)",
             "ms_per_pass": ms_g + ms_c, "gcups_extension": total_cells / (ms_g / 1e3) / 1e9,
             "gcups_cigar_banded": gcells / (ms_c / 1e3) / 1e9}
-        log(f"{name} graph pass: {len(graphs)} extension graphs {ms_g:.3f} ms + {len(ggraphs)} CIGAR graphs "
+        log(f"{name} graph pass: {len(graphs)} extension graphs {ms_g:.3f} ms on {args.graph_streams} streams "
+            f"({timing['one_stream']:.3f} ms on one) + {len(ggraphs)} CIGAR graphs "
             f"({m} tasks) {ms_c:.3f} ms, {mism} CIGAR mismatches")
         del gkeep, gres
     if not args.no_cpu:
@@ -447,6 +467,9 @@ def main():
                     help="other workloads timed after the headline one (cfg4 with its graph-batched CIGAR pass)")
     ap.add_argument("--extra-steps", type=int, default=5)
     ap.add_argument("--graph-batch", type=int, default=262144, help="tasks per hipGraph-captured chunk")
+    ap.add_argument("--graph-streams", type=int, default=4, help="contexts (streams) the chunks' graphs are spread over")
+    ap.add_argument("--graph-in-order", type=int, default=0,
+                    help="capture the chunks' graphs with the launches in order (1) or with side streams (0)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
