@@ -824,6 +824,11 @@ int bsx_timer_stop(bsx_ctx* c, float* ms) {
   return BSX_OK;
 }
 
+// Largest direction slot the host-memory CIGAR call sets by itself: 256 KiB,
+// a task of up to 256 Ki band cells (e.g. a 500-base read with a 201-column
+// band); the scratch is #CU * 192 slots, 12 GB on 256 CUs at this size.
+static constexpr int64_t kZslotHostMax = 1 << 18;
+
 void bsx_set_global_zslot(bsx_ctx* c, int64_t bytes) {
   if (c && bytes >= 64) c->zslot = bytes & ~(int64_t)3;  // whole dwords: the backtrack reads dword-wise
 }
@@ -953,7 +958,21 @@ int bsx_global_batch(bsx_ctx* c, const bsx_scoring* sc, const bsx_glb_tasks* t, 
   int32_t* dr = (int32_t*)c->h_res.p;
   bsx_glb_results drs{r->score ? dr : nullptr, r->n_cigar ? dr + n : nullptr,
                       r->cigar ? (uint32_t*)(dr + 2 * n) : nullptr, r->cigar_slot};
+  // The host sees every task's size, so the direction slot is sized for this
+  // batch: raised (for this call only) to the largest min(qlen, 2w+1) * tlen,
+  // within a scratch budget of kZslotHostMax bytes per slot.
+  const int64_t zslot0 = c->zslot;
+  if (r->cigar) {
+    int64_t need = 0;
+    for (size_t k = 0; k < n; ++k) {
+      const int64_t ql = t->qlen[k], tl = t->tlen[k], w = t->w[k] < 0 ? 0 : t->w[k];
+      const int64_t ncol = ql < 2 * w + 1 ? ql : 2 * w + 1;
+      if (ql > 0 && tl > 0 && ncol * tl > need) need = ncol * tl;
+    }
+    if (need > c->zslot && need <= kZslotHostMax) c->zslot = (need + 3) & ~(int64_t)3;
+  }
   rc = bsx_global_batch_device(c, sc, &dtk, &drs);
+  c->zslot = zslot0;
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(st), "hipStreamSynchronize");
   c->pending_check = false;

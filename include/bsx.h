@@ -151,7 +151,10 @@ int bsx_global_batch_device(bsx_ctx *ctx, const bsx_scoring *sc,
 /* Bytes of direction scratch per in-flight task (default 65536): a CIGAR task
  * needs min(qlen, 2w+1) * tlen bytes; larger ones are rejected (BSX_EINVAL).
  * The context holds #CU * 192 such slots of device memory (#CU * 128 with
- * side streams off, bsx_set_side_streams). */
+ * side streams off, bsx_set_side_streams).  The host-memory call
+ * bsx_global_batch (and so bsx_ksw_global2) raises the slot for its own call
+ * to the batch's largest need, up to 256 KiB; the device-memory call cannot see
+ * the sizes and uses the slot as set. */
 void bsx_set_global_zslot(bsx_ctx *ctx, int64_t bytes);
 
 /* ---- NM, MD:Z and XB:Z of CIGARs (the tail of bwa_gen_cigar2, bwa.c:250-343) ----

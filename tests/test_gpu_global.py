@@ -184,3 +184,39 @@ def test_scalar_ksw_global2_drop_in(engine):
         # score only (n_cigar / cigar NULL, ksw.c:510-513)
         assert L.bsx_ksw_global2(ql, q.ctypes.data, len(t), t.ctypes.data, 5, mat.ctypes.data, 6, 1, 6, 1, w,
                                  None, None) == want_s
+
+
+def test_host_call_sizes_the_direction_slot(engine):
+    """Tasks whose band x target exceeds the default 64 KB direction slot
+    (500-base reads, bands of ~241 columns): the host-memory call and the scalar
+    drop-in size the slot for their own call; the context's slot is unchanged."""
+    import ctypes
+    from bsbolt_amd import bsx
+    from bsbolt_amd.tasks import GlbTasks
+    rng = np.random.default_rng(47)
+    pairs, ws = [], []
+    for _ in range(64):
+        ql = int(rng.integers(450, 520))
+        q = rng.integers(0, 4, ql).astype(np.uint8)
+        t = list(np.where(rng.random(ql) < 0.03, rng.integers(0, 4, ql), q))
+        p = int(rng.integers(0, ql)); t[p:p] = list(rng.integers(0, 4, int(rng.integers(1, 6))))
+        t = np.array(t, np.uint8)
+        pairs.append((q, t)); ws.append(abs(len(t) - ql) + 117)
+    tk = GlbTasks.from_pairs(pairs, ws)
+    need = np.minimum(tk.qlen.astype(np.int64), 2 * tk.w.astype(np.int64) + 1) * tk.tlen
+    assert need.max() > 65536
+    slot = int((tk.qlen + tk.tlen).max())
+    score, ncig, cig = engine.global_(tk, slot=slot)
+    os_, on, oc, _ = O.global2_batch(tk, slot)
+    assert np.array_equal(score, os_) and np.array_equal(ncig, on)
+    for i in range(len(tk)):
+        assert np.array_equal(cig[i, :ncig[i]], oc[i, :on[i]]), f"task {i}"
+    L = bsx.lib()
+    mat = O.fill_scmat(1, 4)
+    q, t = pairs[0]
+    nc = ctypes.c_int(0)
+    cp = ctypes.POINTER(ctypes.c_uint32)()
+    s = L.bsx_ksw_global2(len(q), q.ctypes.data, len(t), t.ctypes.data, 5, mat.ctypes.data, 6, 1, 6, 1, ws[0],
+                          ctypes.byref(nc), ctypes.byref(cp))
+    assert (s, nc.value, [cp[k] for k in range(nc.value)]) == (int(os_[0]), int(on[0]), list(oc[0, :on[0]]))
+    ctypes.CDLL(None).free(ctypes.cast(cp, ctypes.c_void_p))
