@@ -580,6 +580,27 @@ def main():
     if not args.no_learned:
         learned = learned_leg(args, eng, ptrs, outs, cells_d, got, cells, n, total_cells)
 
+    # roofline of the frame kernels the 24-column form runs (class 0: the lane
+    # frame kernel for queries <= 128 bases, the 8x3 group frame kernel above):
+    # useful work = the cells of the tasks whose run the 24-column frame holds
+    # (band label 0 from the GPU's own labelling pass), over the class's kernel
+    # time with the launches in order
+    frame_roof = None
+    if learned and learned.get("fixed24"):
+        labels = eng.band_labels(batch)
+        acc = labels == 0
+        useful = int(cells[acc].sum())
+        f24 = learned["fixed24"]
+        t_ms = f24["kernel_ms_per_step"]["narrow"]
+        tops_f = useful * OPS_PER_CELL / (t_ms / 1e3) / 1e12
+        frame_roof = {"bound": "valu", "kernel": "ext_lane_frame_kernel<24> + ext_narrow_kernel<8, 3> (class 0)",
+                      "achieved": tops_f, "peak": PEAK_PK16_TOPS, "unit": "TOP/s", "frac": tops_f / PEAK_PK16_TOPS,
+                      "frac_int32": tops_f / PEAK_INT32_TOPS, "ms_per_launch": t_ms, "useful_cells": useful,
+                      "tasks_held": int(acc.sum()), "tasks_routed": int(f24["routed"]["f24"]),
+                      "tasks_held_by_stats": int(f24["narrow_accepted"])}
+        log(f"24-column frame kernels: {t_ms:.3f} ms, {useful} useful cells, {tops_f:.2f} TOP/s "
+            f"({tops_f / PEAK_PK16_TOPS:.4f} of packed peak); held {int(acc.sum())} (stats {f24['narrow_accepted']})")
+
     best_form = "plain"
     best_ms = ms_step
     for form, leg in (("learned", learned), ("fixed 24-column frame", (learned or {}).get("fixed24"))):
@@ -618,6 +639,7 @@ def main():
         "roofline_hbm": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes": alg_bytes},
         "learned_band": learned,
+        "roofline_frame24": frame_roof,
         "tuning": {"table": dict(zip(("tpb_plain", "tpb_narrow", "shape_var_65_96", "shape_var_97_128"), tune_table)),
                    "source": tune_src, "device": tune_key},
     }
