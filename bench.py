@@ -488,7 +488,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the extension batch shards "
-                                   "by task index with no exchange (see DESIGN.md), multi-GPU runs are "
+                                   "by task index with no exchange step, multi-GPU runs are "
                                    "not part of this build", "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
@@ -650,7 +650,7 @@ def main():
     del keep, outs, cells_d
     extra = [x for x in args.extra.split(",") if x and x != args.workload]
     line["workloads"] = {x: workload_leg(args, eng, x, cigar=(x == "cfg4")) for x in extra}
-    if learned:  # the north-star ratio: learned-band form over the plain form, and the no-predictor frame form
+    if learned:  # the learned-band form over the plain form, and the no-predictor frame form
         learned["speedup_vs_plain"] = ms_step / learned["ms_per_step"]
         if learned.get("fixed24"):
             learned["fixed24"]["speedup_vs_plain"] = ms_step / learned["fixed24"]["ms_per_step"]
