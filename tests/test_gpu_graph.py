@@ -120,3 +120,33 @@ def test_stream_global_in_graph_batches(engine):
     assert np.array_equal(s, s1) and np.array_equal(nc, nc1) and np.array_equal(cg, cg1)
     ws_, wn, wc, _ = O.global2_batch(tk, cg.shape[1])
     assert np.array_equal(s, ws_) and np.array_equal(nc, wn)
+
+
+def test_stream_cfg4_cigar_pass_full_graph_batch(engine):
+    """The RRBS workload's CIGAR pass (every try of mem_reg2aln's band doubling,
+    built in rounds on the GPU) streamed through hipGraph-captured batches of
+    262,144 tasks: at least one full batch plus an eager remainder, equal to the
+    oracle on every task (scores, CIGAR counts, CIGARs)."""
+    from bsbolt_amd.tasks import GlbTasks
+    from bsbolt_amd.workload import CONFIGS, Workload, build_glb_batch
+    wl = Workload(CONFIGS["cfg4"].scaled(0.05))
+    try:
+        wl.build_ext_batch(lambda t: engine.extend(t, cells=False))
+        gt = build_glb_batch(wl, lambda t: engine.global_(t))[0]
+    finally:
+        wl.close()
+    m = len(gt)
+    reps = -(-300000 // m)  # enough copies for one full 262,144-task graph batch and a remainder
+    qn, tn = len(gt.query), len(gt.target)
+    tk = GlbTasks(np.tile(gt.query, reps), np.concatenate([gt.q_off + np.uint64(r * qn) for r in range(reps)]),
+                  np.tile(gt.qlen, reps), np.tile(gt.target, reps),
+                  np.concatenate([gt.t_off + np.uint64(r * tn) for r in range(reps)]), np.tile(gt.tlen, reps),
+                  np.tile(gt.w, reps))
+    n = len(tk)
+    assert n >= 262144 and n % 262144 != 0
+    print(f"cfg4 CIGAR pass: {m} tasks x {reps} = {n}, graph batches of 262144", flush=True)
+    s, nc, cg = engine.stream_global(tk, batch=262144)
+    ws_, wn, wc, _ = O.global2_batch(tk, cg.shape[1], nthreads=16)
+    assert np.array_equal(s, ws_) and np.array_equal(nc, wn)
+    bad = [i for i in range(n) if not np.array_equal(cg[i, :nc[i]], wc[i, :wn[i]])]
+    assert not bad, f"{len(bad)} CIGARs differ, first {bad[:5]}"
